@@ -1,0 +1,186 @@
+/*
+ * qesmc.h -- C ABI of the MI355X mass-consistency solver (libqesmc.so).
+ *
+ * The library replaces the body of QES-Winds' Solver::solve for one time step:
+ *   R = divergence of u0 (src/winds/Solver_CPU_RB.cpp:56-80),
+ *   red-black SOR for lambda (Solver_CPU_RB.cpp:95-184),
+ *   u = u0 + grad(lambda)/(2 alpha^2) and solid zeroing (Solver_CPU_RB.cpp:194-284),
+ * on one AMD Instinct MI355X (gfx950).  Plain C types only; host pointers are
+ * borrowed for the duration of a call.  Every entry point returns QESMC_OK (0)
+ * or a negative status; qesmc_status_string() names it.  A context is not
+ * thread-safe: use one per host thread.
+ *
+ * Layouts are the reference's (src/qes/Domain.h:143-175): nx = ncx+1,
+ * ny = ncy+1, nz = ncz+2; cell arrays (nx-1)*(ny-1)*(nz-1) and face arrays
+ * nx*ny*nz, x fastest; dz_array has nz-1 entries with [0] the ground ghost.
+ */
+#ifndef QESMC_H
+#define QESMC_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QESMC_ABI_VERSION 1
+
+/* ---- status codes ---------------------------------------------------------
+ * The reference exits the process on every error (Solver_GPU_GlobalMemory.cu:
+ * 58-65 _cudaCheck, SolverFactory.cpp:57-58 QESout::error); the library
+ * returns these codes instead and the C++ adapter maps them back to exits. */
+#define QESMC_OK 0
+#define QESMC_ERR_ARG (-1)          /* null pointer, bad size, dx != dy ...     */
+#define QESMC_ERR_NO_DEVICE (-2)    /* no HIP device / bad ordinal              */
+#define QESMC_ERR_OOM (-3)          /* device allocation failed                 */
+#define QESMC_ERR_HIP (-4)          /* other HIP runtime error                  */
+#define QESMC_ERR_ZERO_DIAG (-5)    /* an updated cell has e+f+g+h+m+n == 0     */
+#define QESMC_ERR_NONFINITE (-6)    /* lambda became NaN/Inf during the solve   */
+#define QESMC_ERR_STATE (-7)        /* resident call before its inputs were set */
+#define QESMC_ERR_UNSUPPORTED (-8)  /* option not available for these inputs   */
+
+/* ---- grid / input / output descriptors ---------------------------------- */
+
+/* qes::Domain sizes (Domain.cpp:7-31).  dy is accepted for completeness; the
+ * reference's coefficients are normalised with dx in both horizontal
+ * directions (Domain.h:104 getDomainSize quirk) and its R/correction use
+ * domain.dy(); the library uses dx and dy exactly where the reference does. */
+typedef struct qesmc_grid {
+  int nx, ny, nz;          /* staggered sizes                                */
+  float dx, dy;            /* horizontal spacing [m]                         */
+  const float *dz_array;   /* nz-1 vertical spacings [m], [0] = ghost        */
+} qesmc_grid;
+
+/* WINDSGeneralData::e,f,g,h,m,n and icellflag (WINDSGeneralData.h:231-239). */
+typedef struct qesmc_coeffs {
+  const float *e, *f, *g, *h, *m, *n; /* cell arrays, already normalised     */
+  const int *icellflag;               /* cell array: 0 building, 1 fluid, 2 terrain */
+} qesmc_coeffs;
+
+/* WINDSGeneralData::u0,v0,w0 (inputs) and u,v,w (outputs), face arrays. */
+typedef struct qesmc_fields {
+  const float *u0, *v0, *w0;
+  float *u, *v, *w;
+} qesmc_fields;
+
+/* Kernel path selection (qesmc_params.path). */
+#define QESMC_PATH_AUTO 0     /* fastest available                              */
+#define QESMC_PATH_PLAIN 1    /* in-place red/black launches on the x-fastest
+                                 layout, full fp32 coefficients: the classic
+                                 GPU red-black SOR (baseline)                   */
+#define QESMC_PATH_SPLIT 2    /* colour-split (checkerboard) layout, per-cell
+                                 6-bit wall mask when the coefficients are
+                                 stair-step, full coefficients otherwise        */
+
+typedef struct qesmc_params {
+  float omega;      /* over-relaxation, reference 1.78f (Solver.h:83)          */
+  float tol;        /* stop when max|dlambda| <= tol (Solver_CPU_RB.cpp:95)    */
+  int itermax;      /* iteration cap (SimulationParameters maxIterations)      */
+  int alpha1;       /* Gaussian precision moduli, reference 1 (Solver.cpp:79)  */
+  int alpha2;
+  int path;         /* QESMC_PATH_*                                            */
+  int check_every;  /* host polls convergence every this many iterations (>=1);
+                       results never depend on it                             */
+  int flags;        /* QESMC_RUN_* bits                                        */
+} qesmc_params;
+
+/* qesmc_params.flags: rebuild the stencil encoding from the resident
+ * coefficients inside qesmc_run (so its cost is part of the run's time). */
+#define QESMC_RUN_REENCODE 1
+
+typedef struct qesmc_result {
+  int iterations;            /* SOR iterations executed (reference prints it, :192) */
+  int path;                  /* path that ran (QESMC_PATH_PLAIN / _SPLIT)         */
+  int compressed;            /* 1 if the wall-mask stencil was used               */
+  int status;                /* same as the return code                           */
+  double final_max_dlambda;  /* max|dlambda| of the last iteration                */
+  double ms_total;           /* device time of R + SOR + correction (HIP events)  */
+  double ms_sor;             /* device time of the SOR loop alone                 */
+  double glups;              /* (nx-3)(ny-3)(nz-3) * iterations / ms_sor          */
+} qesmc_result;
+
+typedef struct qesmc_ctx qesmc_ctx;
+
+/* ---- lifetime -------------------------------------------------------------- */
+
+/* Create a context on HIP device `device` with its own stream.  Replaces the
+ * device selection and per-solve allocation of Solver_GPU_GlobalMemory
+ * (Solver_GPU_GlobalMemory.cu:201-268); buffers persist across solves. */
+int qesmc_create(qesmc_ctx **out, int device);
+int qesmc_destroy(qesmc_ctx *ctx);
+
+/* Reference defaults: omega 1.78f, tol 1e-9f, itermax 500, alpha 1,
+ * path AUTO, check_every 64, flags 0. */
+void qesmc_default_params(qesmc_params *p);
+
+const char *qesmc_status_string(int status);
+
+/* ---- one-call drop-in ------------------------------------------------------ */
+
+/* Solver::solve(WINDSGeneralData*, const int& itermax) (Solver.h:112): copies
+ * inputs to the device, runs R + SOR + correction, copies u, v, w back.
+ * lambda_inout: NULL for a cold start (Solver::resetLambda, Solver.h:115-120),
+ * else the (nx-1)(ny-1)(nz-1) initial lambda, overwritten with the final one
+ * (keeps Solver::lambda in step, so the CPU solvers' warm start across time
+ * steps in qes/qesMain.cpp:136-150 is preserved). */
+int qesmc_solve(qesmc_ctx *ctx, const qesmc_grid *grid, const qesmc_coeffs *coeffs,
+                qesmc_fields *fields, float *lambda_inout, const qesmc_params *params,
+                qesmc_result *result);
+
+/* ---- resident interface (inputs stay in HBM across solves) ----------------- */
+
+/* Upload grid + coefficients + flags (the WINDSGeneralData constructor's
+ * products, WINDSGeneralData.cpp:859-866) and build the stencil encoding. */
+int qesmc_set_geometry(qesmc_ctx *ctx, const qesmc_grid *grid, const qesmc_coeffs *coeffs);
+/* Upload u0, v0, w0 (applyWindProfile + applyParametrizations products). */
+int qesmc_set_inflow(qesmc_ctx *ctx, const float *u0, const float *v0, const float *w0);
+/* Initial lambda for the next run; NULL = zeros (resetLambda). */
+int qesmc_set_lambda(qesmc_ctx *ctx, const float *lambda);
+/* R + SOR + correction on the resident inputs; u, v, w and lambda stay on
+ * the device.  The next run starts from this run's lambda unless
+ * qesmc_set_lambda is called in between. */
+int qesmc_run(qesmc_ctx *ctx, const qesmc_params *params, qesmc_result *result);
+int qesmc_get_velocity(qesmc_ctx *ctx, float *u, float *v, float *w);
+int qesmc_get_lambda(qesmc_ctx *ctx, float *lambda);
+/* Max |div| over interior fluid cells of the corrected field, FP64 on the
+ * device: physical = 0 uses the reference operator R(u)/(-2 alpha1^2),
+ * physical = 1 the dimensionally consistent one (vertical term / dz_k). */
+int qesmc_divergence_linf(qesmc_ctx *ctx, int physical, double *out);
+
+/* ---- input builder (flat stair-step cities) ------------------------------ */
+
+/* One RectangularBuilding (RectangularBuilding.h:69-110). */
+typedef struct qesmc_building {
+  float x_start, y_start, length, width, height, base_height, rotation_deg;
+} qesmc_building;
+
+/* One sensor profile (WindProfilerSensorType.cpp:290-333):
+ * kind 0 uniform, 1 log law, 2 power law (exponent in z0_or_p, as siteZ0). */
+typedef struct qesmc_inflow {
+  int kind;
+  float U_ref, z_ref, z0_or_p, wind_dir_deg;
+} qesmc_inflow;
+
+/* Build the WINDSGeneralData arrays of a flat city: ground ghost flags,
+ * stair-step building flags (PolyBuilding.cpp:130-236), wall coefficients
+ * (Wall.cpp:44-349, 827-846), single-sensor u0/v0/w0 (WindProfilerSensorType.cpp
+ * :571-621) and solid zeroing (Wall.cpp:805-824).  Outputs are caller-owned
+ * arrays of the sizes above. */
+int qesmc_city_build(int nx, int ny, int nz, float dx, const float *dz_array, int nbuildings,
+                     const qesmc_building *buildings, const qesmc_inflow *inflow, float *e,
+                     float *f, float *g, float *h, float *m, float *n, int *icellflag, float *u0,
+                     float *v0, float *w0);
+
+/* Seeded non-overlapping rectangles: footprints U(lmin,lmax) m, heights
+ * U(hmin,hmax) m (or lognormal(median hmin, sigma hmax) capped at hcap when
+ * lognormal != 0), one free cell between buildings and `margin` cells from
+ * the lateral boundaries.  Returns the number placed (<= nbuildings). */
+int qesmc_city_random_buildings(unsigned long long seed, int nx, int ny, float dx, int nbuildings,
+                                float lmin, float lmax, float hmin, float hmax, int lognormal,
+                                float hcap, int margin, qesmc_building *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QESMC_H */

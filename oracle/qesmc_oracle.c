@@ -1,0 +1,672 @@
+/*
+ * qesmc_oracle.c -- CPU restatement of QES-Winds' mass-consistency step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.  The
+ * product (libqesmc, HIP) never links, loads or calls anything here.
+ *
+ * Every function restates one piece of the reference (UtahEFD/QES-Public,
+ * CMake project 2.3.0) with the same loop nests, bounds, evaluation order and
+ * float/double promotions, so that with -ffp-contract=off on x86-64 (SSE,
+ * FLT_EVAL_METHOD == 0, no FMA) the fp32 paths reproduce the reference's
+ * arithmetic operation for operation.  Citations are file:line in the
+ * reference tree.
+ *
+ * Index conventions (src/qes/Domain.h:143-175):
+ *   nx = ncx+1, ny = ncy+1, nz = ncz+2 (staggered sizes, Domain.cpp:12-14)
+ *   cell(i,j,k) = i + j*(nx-1) + k*(nx-1)*(ny-1)        (cell-centred arrays)
+ *   face(i,j,k) = i + j*nx + k*nx*ny                     (face arrays)
+ *
+ * Pinning status of each function is recorded in the header comment of the
+ * function and in tests/test_oracle_*.py.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define CELL(i, j, k) ((long)(i) + (long)(j) * (nx - 1) + (long)(k) * (long)(nx - 1) * (ny - 1))
+#define FACE(i, j, k) ((long)(i) + (long)(j) * nx + (long)(k) * (long)nx * ny)
+
+/* ------------------------------------------------------------------------ */
+/* a6: divergence source R                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* Solver_CPU_RB.cpp:56-80 (identical in Solver_CPU.cpp:56-77).
+ * Quirk A1 kept: the vertical term is m*dz_k*0.5*(dz_k+dz_{k+1})*w0, i.e. the
+ * plain difference of w0, and its first product runs in double because of the
+ * double literal 0.5.  Only layers k in [1, nz-3] are written. */
+void oracle_divergence_f32(int nx, int ny, int nz, float dx, float dy, const float *dz_array,
+                           const float *e, const float *f, const float *g, const float *h,
+                           const float *m, const float *n, const float *u0, const float *v0,
+                           const float *w0, int alpha1, float *R)
+{
+  const double pre = -2.0f * pow(alpha1, 2.0);
+#pragma omp parallel for schedule(static)
+  for (int k = 1; k < nz - 2; ++k) {
+    for (int j = 0; j < ny - 1; ++j) {
+      for (int i = 0; i < nx - 1; ++i) {
+        long c = CELL(i, j, k);
+        long fc = FACE(i, j, k);
+        R[c] = pre
+               * (((e[c] * u0[fc + 1] - f[c] * u0[fc]) * dx)
+                  + ((g[c] * v0[fc + nx] - h[c] * v0[fc]) * dy)
+                  + ((m[c] * dz_array[k] * 0.5 * (dz_array[k] + dz_array[k + 1]) * w0[fc + (long)nx * ny]
+                      - n[c] * dz_array[k] * 0.5f * (dz_array[k] + dz_array[k - 1]) * w0[fc])));
+      }
+    }
+  }
+}
+
+/* FP64 restatement of the same operator: every operation of
+ * Solver_CPU_RB.cpp:56-80 carried out in double on the (exactly promoted)
+ * fp32 inputs.  This defines the discrete problem the FP64 converged oracle
+ * solves. */
+void oracle_divergence_f64(int nx, int ny, int nz, double dx, double dy, const float *dz_array,
+                           const float *e, const float *f, const float *g, const float *h,
+                           const float *m, const float *n, const float *u0, const float *v0,
+                           const float *w0, int alpha1, double *R)
+{
+  const double pre = -2.0 * pow(alpha1, 2.0);
+#pragma omp parallel for schedule(static)
+  for (int k = 1; k < nz - 2; ++k) {
+    for (int j = 0; j < ny - 1; ++j) {
+      for (int i = 0; i < nx - 1; ++i) {
+        long c = CELL(i, j, k);
+        long fc = FACE(i, j, k);
+        double dzk = dz_array[k], dzp = dz_array[k + 1], dzm = dz_array[k - 1];
+        R[c] = pre
+               * ((((double)e[c] * u0[fc + 1] - (double)f[c] * u0[fc]) * dx)
+                  + (((double)g[c] * v0[fc + nx] - (double)h[c] * v0[fc]) * dy)
+                  + (((double)m[c] * dzk * 0.5 * (dzk + dzp) * w0[fc + (long)nx * ny]
+                      - (double)n[c] * dzk * 0.5 * (dzk + dzm) * w0[fc])));
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* a7-a11: SOR loops                                                          */
+/* ------------------------------------------------------------------------ */
+
+/* One red-black colour pass, Solver_CPU_RB.cpp:109-129 (colour 0 = red) and
+ * :132-151 (colour 1 = black).  Expression kept operand for operand. */
+static void rb_pass_f32(int nx, int ny, int nz, int colour, const float *e, const float *f,
+                        const float *g, const float *h, const float *m, const float *n,
+                        const float *R, float *lambda, float omega)
+{
+  const long sy = nx - 1, sz = (long)(nx - 1) * (ny - 1);
+#pragma omp for schedule(static)
+  for (int k = 1; k < nz - 2; ++k) {
+    for (int j = 1; j < ny - 2; ++j) {
+      for (int i = 1; i < nx - 2; ++i) {
+        if (((i + j + k) % 2) == colour) {
+          long c = CELL(i, j, k);
+          lambda[c] = (omega / (e[c] + f[c] + g[c] + h[c] + m[c] + n[c]))
+                        * (e[c] * lambda[c + 1] + f[c] * lambda[c - 1] + g[c] * lambda[c + sy]
+                           + h[c] * lambda[c - sy] + m[c] * lambda[c + sz] + n[c] * lambda[c - sz]
+                           - R[c])
+                      + (1.0f - omega) * lambda[c];
+        }
+      }
+    }
+  }
+}
+
+/* Solver_CPU_RB::solve SOR loop, Solver_CPU_RB.cpp:88-184:
+ *   while (iter < itermax && max_error > tol) {
+ *     lambda_old = lambda (:100-104); red (:109-129); black (:132-151);
+ *     mirror lambda(k=0) = lambda(k=1) (:154-162);
+ *     max_error = max over k in [1,nz-2], all i,j of fabs(lambda-lambda_old) (:164-180)
+ *   }
+ * lambda_old is caller-provided scratch of the same size.  Returns the
+ * iteration count (printed, not returned, by the reference, :192) and stores
+ * the last max_error.  OpenMP work-sharing as in the reference; the result is
+ * independent of the thread count because updates within a colour are
+ * independent. */
+int oracle_sor_rb_f32(int nx, int ny, int nz, const float *e, const float *f, const float *g,
+                      const float *h, const float *m, const float *n, const float *R,
+                      float *lambda, float *lambda_old, float omega, float tol, int itermax,
+                      float *max_error_out)
+{
+  int iter = 0;
+  float max_error = 1.0f;
+  const long ncell = (long)(nx - 1) * (ny - 1) * (nz - 1);
+  while (iter < itermax && max_error > tol) {
+    float me = 0.0f;
+#pragma omp parallel
+    {
+#pragma omp for schedule(static)
+      for (long q = 0; q < ncell; ++q) lambda_old[q] = lambda[q];
+      rb_pass_f32(nx, ny, nz, 0, e, f, g, h, m, n, R, lambda, omega);
+      rb_pass_f32(nx, ny, nz, 1, e, f, g, h, m, n, R, lambda, omega);
+#pragma omp for schedule(static)
+      for (int j = 0; j < ny - 1; ++j)
+        for (int i = 0; i < nx - 1; ++i) lambda[CELL(i, j, 0)] = lambda[CELL(i, j, 1)];
+#pragma omp for schedule(static) reduction(max : me)
+      for (int k = 1; k < nz - 1; ++k)
+        for (int j = 0; j < ny - 1; ++j)
+          for (int i = 0; i < nx - 1; ++i) {
+            long c = CELL(i, j, k);
+            float err = fabsf(lambda[c] - lambda_old[c]);
+            if (err > me) me = err;
+          }
+    }
+    max_error = me;
+    iter += 1;
+  }
+  if (max_error_out) *max_error_out = max_error;
+  return iter;
+}
+
+/* Solver_CPU::solve SOR loop, Solver_CPU.cpp:88-140: one in-place
+ * lexicographic sweep (k, then j, then i) per iteration, then the mirror and
+ * the same max|dlambda| stop test.  Single-threaded as in the reference. */
+int oracle_sor_serial_f32(int nx, int ny, int nz, const float *e, const float *f, const float *g,
+                          const float *h, const float *m, const float *n, const float *R,
+                          float *lambda, float *lambda_old, float omega, float tol, int itermax,
+                          float *max_error_out)
+{
+  int iter = 0;
+  float max_error = 1.0f;
+  const long sy = nx - 1, sz = (long)(nx - 1) * (ny - 1);
+  const long ncell = (long)(nx - 1) * (ny - 1) * (nz - 1);
+  while (iter < itermax && max_error > tol) {
+    memcpy(lambda_old, lambda, sizeof(float) * ncell);
+    for (int k = 1; k < nz - 2; ++k)
+      for (int j = 1; j < ny - 2; ++j)
+        for (int i = 1; i < nx - 2; ++i) {
+          long c = CELL(i, j, k);
+          lambda[c] = (omega / (e[c] + f[c] + g[c] + h[c] + m[c] + n[c]))
+                        * (e[c] * lambda[c + 1] + f[c] * lambda[c - 1] + g[c] * lambda[c + sy]
+                           + h[c] * lambda[c - sy] + m[c] * lambda[c + sz] + n[c] * lambda[c - sz]
+                           - R[c])
+                      + (1.0f - omega) * lambda[c];
+        }
+    for (int j = 0; j < ny - 1; ++j)
+      for (int i = 0; i < nx - 1; ++i) lambda[CELL(i, j, 0)] = lambda[CELL(i, j, 1)];
+    max_error = 0.0f;
+    for (int k = 1; k < nz - 1; ++k)
+      for (int j = 0; j < ny - 1; ++j)
+        for (int i = 0; i < nx - 1; ++i) {
+          long c = CELL(i, j, k);
+          float err = fabsf(lambda[c] - lambda_old[c]);
+          if (err > max_error) max_error = err;
+        }
+    iter += 1;
+  }
+  if (max_error_out) *max_error_out = max_error;
+  return iter;
+}
+
+/* FP64 red-black SOR: the loop of Solver_CPU_RB.cpp:95-184 with lambda, R,
+ * the relaxation arithmetic and the stop test in double.  Used as the
+ * converged oracle (tol 1e-9 reachable in double).  Coefficients stay the
+ * caller's fp32 values, promoted exactly. */
+int oracle_sor_rb_f64(int nx, int ny, int nz, const float *e, const float *f, const float *g,
+                      const float *h, const float *m, const float *n, const double *R,
+                      double *lambda, double *lambda_old, double omega, double tol, int itermax,
+                      double *max_error_out)
+{
+  int iter = 0;
+  double max_error = 1.0;
+  const long sy = nx - 1, sz = (long)(nx - 1) * (ny - 1);
+  const long ncell = (long)(nx - 1) * (ny - 1) * (nz - 1);
+  while (iter < itermax && max_error > tol) {
+    double me = 0.0;
+#pragma omp parallel
+    {
+#pragma omp for schedule(static)
+      for (long q = 0; q < ncell; ++q) lambda_old[q] = lambda[q];
+      for (int colour = 0; colour < 2; ++colour) {
+#pragma omp for schedule(static)
+        for (int k = 1; k < nz - 2; ++k)
+          for (int j = 1; j < ny - 2; ++j)
+            for (int i = 1; i < nx - 2; ++i) {
+              if (((i + j + k) % 2) != colour) continue;
+              long c = CELL(i, j, k);
+              double ec = e[c], fc = f[c], gc = g[c], hc = h[c], mc = m[c], nc = n[c];
+              lambda[c] = (omega / (ec + fc + gc + hc + mc + nc))
+                            * (ec * lambda[c + 1] + fc * lambda[c - 1] + gc * lambda[c + sy]
+                               + hc * lambda[c - sy] + mc * lambda[c + sz] + nc * lambda[c - sz]
+                               - R[c])
+                          + (1.0 - omega) * lambda[c];
+            }
+      }
+#pragma omp for schedule(static)
+      for (int j = 0; j < ny - 1; ++j)
+        for (int i = 0; i < nx - 1; ++i) lambda[CELL(i, j, 0)] = lambda[CELL(i, j, 1)];
+#pragma omp for schedule(static) reduction(max : me)
+      for (int k = 1; k < nz - 1; ++k)
+        for (int j = 0; j < ny - 1; ++j)
+          for (int i = 0; i < nx - 1; ++i) {
+            long c = CELL(i, j, k);
+            double err = fabs(lambda[c] - lambda_old[c]);
+            if (err > me) me = err;
+          }
+    }
+    max_error = me;
+    iter += 1;
+  }
+  if (max_error_out) *max_error_out = max_error;
+  return iter;
+}
+
+/* ------------------------------------------------------------------------ */
+/* a12-a13: velocity correction and solid zeroing                            */
+/* ------------------------------------------------------------------------ */
+
+/* Solver_CPU_RB.cpp:194-284: u = u0 (all faces), then for k in [1,nz-3],
+ * j in [1,ny-2], i in [1,nx-2]:
+ *   u = u0 + (1/(2 a1^2)) * f * dx * (lambda_c - lambda_{i-1})    (:214-227)
+ *   v = v0 + (1/(2 a1^2)) * h * dy * (lambda_c - lambda_{j-1})    (:230-243)
+ *   w = w0 + (1/(2 a2^2)) * n * dz_k * (lambda_c - lambda_{k-1})  (:246-259)
+ * then every face of a cell with icellflag 0 or 2 (k in [1,nz-2]) is zeroed
+ * (:262-284). */
+void oracle_velocity_f32(int nx, int ny, int nz, float dx, float dy, const float *dz_array,
+                         const float *f, const float *h, const float *n, const int *icellflag,
+                         const float *u0, const float *v0, const float *w0, const float *lambda,
+                         int alpha1, int alpha2, float *u, float *v, float *w)
+{
+  const long nface = (long)nx * ny * nz;
+  const long sy = nx - 1, sz = (long)(nx - 1) * (ny - 1);
+  const float p1 = 1.0f / (2.0f * (float)pow(alpha1, 2.0));
+  const float p2 = 1.0f / (2.0f * (float)pow(alpha2, 2.0));
+#pragma omp parallel
+  {
+#pragma omp for schedule(static)
+    for (long q = 0; q < nface; ++q) {
+      u[q] = u0[q];
+      v[q] = v0[q];
+      w[q] = w0[q];
+    }
+#pragma omp for schedule(static)
+    for (int k = 1; k < nz - 2; ++k)
+      for (int j = 1; j < ny - 1; ++j)
+        for (int i = 1; i < nx - 1; ++i) {
+          long c = CELL(i, j, k), fc = FACE(i, j, k);
+          u[fc] = u0[fc] + p1 * f[c] * dx * (lambda[c] - lambda[c - 1]);
+          v[fc] = v0[fc] + p1 * h[c] * dy * (lambda[c] - lambda[c - sy]);
+          w[fc] = w0[fc] + p2 * n[c] * dz_array[k] * (lambda[c] - lambda[c - sz]);
+        }
+#pragma omp for schedule(static)
+    for (int k = 1; k < nz - 1; ++k)
+      for (int j = 0; j < ny - 1; ++j)
+        for (int i = 0; i < nx - 1; ++i) {
+          long c = CELL(i, j, k), fc = FACE(i, j, k);
+          if (icellflag[c] == 0 || icellflag[c] == 2) {
+            u[fc] = 0;
+            u[fc + 1] = 0;
+            v[fc] = 0;
+            v[fc + nx] = 0;
+            w[fc] = 0;
+            w[fc + (long)nx * ny] = 0;
+          }
+        }
+  }
+}
+
+/* The same correction and zeroing in double, from an FP64 lambda
+ * (Solver_CPU_RB.cpp:194-284 with every operation in double). */
+void oracle_velocity_f64(int nx, int ny, int nz, double dx, double dy, const float *dz_array,
+                         const float *f, const float *h, const float *n, const int *icellflag,
+                         const float *u0, const float *v0, const float *w0, const double *lambda,
+                         int alpha1, int alpha2, double *u, double *v, double *w)
+{
+  const long nface = (long)nx * ny * nz;
+  const long sy = nx - 1, sz = (long)(nx - 1) * (ny - 1);
+  const double p1 = 1.0 / (2.0 * pow(alpha1, 2.0));
+  const double p2 = 1.0 / (2.0 * pow(alpha2, 2.0));
+#pragma omp parallel
+  {
+#pragma omp for schedule(static)
+    for (long q = 0; q < nface; ++q) {
+      u[q] = u0[q];
+      v[q] = v0[q];
+      w[q] = w0[q];
+    }
+#pragma omp for schedule(static)
+    for (int k = 1; k < nz - 2; ++k)
+      for (int j = 1; j < ny - 1; ++j)
+        for (int i = 1; i < nx - 1; ++i) {
+          long c = CELL(i, j, k), fc = FACE(i, j, k);
+          u[fc] = u0[fc] + p1 * f[c] * dx * (lambda[c] - lambda[c - 1]);
+          v[fc] = v0[fc] + p1 * h[c] * dy * (lambda[c] - lambda[c - sy]);
+          w[fc] = w0[fc] + p2 * n[c] * (double)dz_array[k] * (lambda[c] - lambda[c - sz]);
+        }
+#pragma omp for schedule(static)
+    for (int k = 1; k < nz - 1; ++k)
+      for (int j = 0; j < ny - 1; ++j)
+        for (int i = 0; i < nx - 1; ++i) {
+          long c = CELL(i, j, k), fc = FACE(i, j, k);
+          if (icellflag[c] == 0 || icellflag[c] == 2) {
+            u[fc] = 0;
+            u[fc + 1] = 0;
+            v[fc] = 0;
+            v[fc + nx] = 0;
+            w[fc] = 0;
+            w[fc + (long)nx * ny] = 0;
+          }
+        }
+  }
+}
+
+/* Whole Solver_CPU_RB::solve (Solver_CPU_RB.cpp:41-291) for convenience:
+ * R, SOR loop, correction.  R, lambda_old are caller scratch; lambda is
+ * in/out (warm start when non-zero, as when resetLambda is not called). */
+int oracle_solve_rb_f32(int nx, int ny, int nz, float dx, float dy, const float *dz_array,
+                        const float *e, const float *f, const float *g, const float *h,
+                        const float *m, const float *n, const int *icellflag, const float *u0,
+                        const float *v0, const float *w0, float *R, float *lambda,
+                        float *lambda_old, float omega, float tol, int itermax, int alpha1,
+                        int alpha2, float *u, float *v, float *w, float *max_error_out)
+{
+  oracle_divergence_f32(nx, ny, nz, dx, dy, dz_array, e, f, g, h, m, n, u0, v0, w0, alpha1, R);
+  int it = oracle_sor_rb_f32(nx, ny, nz, e, f, g, h, m, n, R, lambda, lambda_old, omega, tol,
+                             itermax, max_error_out);
+  oracle_velocity_f32(nx, ny, nz, dx, dy, dz_array, f, h, n, icellflag, u0, v0, w0, lambda, alpha1,
+                      alpha2, u, v, w);
+  return it;
+}
+
+/* Divergence of a face field measured with the reference operator, i.e.
+ * R(u)/(-2 a1^2) of Solver_CPU_RB.cpp:56-80 evaluated in double on the
+ * corrected field, over fluid cells (icellflag 1) of the updated interior
+ * k in [1,nz-3], j in [1,ny-3], i in [1,nx-3].  Returns the L-infinity norm.
+ * With physical != 0 the vertical term is divided by dz_k (the physically
+ * consistent divergence, quirk A1 removed). */
+double oracle_div_linf(int nx, int ny, int nz, double dx, double dy, const float *dz_array,
+                       const float *e, const float *f, const float *g, const float *h,
+                       const float *m, const float *n, const int *icellflag, const float *u,
+                       const float *v, const float *w, int physical)
+{
+  double mx = 0.0;
+#pragma omp parallel for schedule(static) reduction(max : mx)
+  for (int k = 1; k < nz - 2; ++k)
+    for (int j = 1; j < ny - 2; ++j)
+      for (int i = 1; i < nx - 2; ++i) {
+        long c = CELL(i, j, k), fc = FACE(i, j, k);
+        if (icellflag[c] != 1) continue;
+        double d;
+        if (physical) {
+          d = ((double)u[fc + 1] - u[fc]) / dx + ((double)v[fc + nx] - v[fc]) / dy
+              + ((double)w[fc + (long)nx * ny] - w[fc]) / dz_array[k];
+        } else {
+          double dzk = dz_array[k];
+          d = (((double)e[c] * u[fc + 1] - (double)f[c] * u[fc]) * dx)
+              + (((double)g[c] * v[fc + nx] - (double)h[c] * v[fc]) * dy)
+              + ((double)m[c] * dzk * 0.5 * (dzk + dz_array[k + 1]) * w[fc + (long)nx * ny]
+                 - (double)n[c] * dzk * 0.5 * (dzk + dz_array[k - 1]) * w[fc]);
+        }
+        d = fabs(d);
+        if (d > mx) mx = d;
+      }
+  return mx;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Inputs of the path (a1, a3, a4, a5): grid, stair-step flags, wall         */
+/* coefficients, single-sensor inflow profile.                               */
+/* ------------------------------------------------------------------------ */
+
+/* Domain::defineVerticalGrid, Domain.cpp:146-160.  dz_array has nz-1
+ * entries; z_face nz entries; z nz-1 entries. */
+void oracle_vertical_grid(int nz, const float *dz_array, float *z_face, float *z)
+{
+  z_face[0] = -dz_array[0];
+  z_face[1] = 0.0f;
+  for (int k = 2; k < nz; ++k) z_face[k] = z_face[k - 1] + dz_array[k - 1];
+  z[0] = -0.5f * dz_array[0];
+  for (int k = 1; k < nz - 1; ++k) z[k] = 0.5f * (z_face[k] + z_face[k + 1]);
+}
+
+/* WINDSGeneralData::allocateMemory (WINDSGeneralData.cpp:1077-1095: e..n = 1,
+ * icellflag = 1) and the ground ghost layer icellflag(k=0) = 2
+ * (WINDSGeneralData.cpp:439-444). */
+void oracle_init_cells(int nx, int ny, int nz, float *e, float *f, float *g, float *h, float *m,
+                       float *n, int *icellflag)
+{
+  const long ncell = (long)(nx - 1) * (ny - 1) * (nz - 1);
+  for (long q = 0; q < ncell; ++q) {
+    e[q] = f[q] = g[q] = h[q] = m[q] = n[q] = 1.0f;
+    icellflag[q] = 1;
+  }
+  for (int j = 0; j < ny - 1; ++j)
+    for (int i = 0; i < nx - 1; ++i) icellflag[CELL(i, j, 0)] = 2;
+}
+
+/* One rectangular building: RectangularBuilding::parseValues vertex list
+ * (RectangularBuilding.h:86-103, rotation in degrees) followed by
+ * PolyBuilding::setCellFlags with meshTypeFlag 0 (PolyBuilding.cpp:130-236):
+ * bounding-box index ranges, k_start/k_end searches over z_face / z, and a
+ * PNPOLY crossing test at cell centres.  Cells inside get icellflag 0. */
+void oracle_rect_building(int nx, int ny, int nz, float dx, float dy_unused, const float *z_face,
+                          const float *z, float x_start, float y_start, float L, float W,
+                          float H, float base_height, float rotation_deg, int *icellflag)
+{
+  (void)dy_unused;
+  /* getDomainSize() returns (dx, dx, dz): dy is dx (Domain.h:104). */
+  const float dy = dx;
+  float height_eff = H + base_height;
+  double rot = rotation_deg * M_PI / 180.0;
+  float rotf = (float)rot;
+  float xv[5], yv[5];
+  xv[0] = xv[4] = x_start;
+  yv[0] = yv[4] = y_start;
+  /* sin/cos of a float resolve to the float overloads in the reference (C++) */
+  xv[1] = x_start - W * sinf(rotf);
+  yv[1] = y_start + W * cosf(rotf);
+  xv[2] = xv[1] + L * cosf(rotf);
+  yv[2] = yv[1] + L * sinf(rotf);
+  xv[3] = x_start + L * cosf(rotf);
+  yv[3] = y_start + L * sinf(rotf);
+
+  float x_min = xv[0], x_max = xv[0], y_min = yv[0], y_max = yv[0];
+  for (int id = 1; id < 5; ++id) {
+    if (xv[id] > x_max) x_max = xv[id];
+    if (xv[id] < x_min) x_min = xv[id];
+    if (yv[id] > y_max) y_max = yv[id];
+    if (yv[id] < y_min) y_min = yv[id];
+  }
+  int i_start = (int)(x_min / dx);
+  int i_end = (int)((x_max / dx) + 1);
+  int j_start = (int)(y_min / dy);
+  int j_end = (int)((y_max / dy) + 1);
+
+  const int zsize = nz - 1;
+  int k_start = 1, k_end = 1;
+  for (int k = 1; k < zsize; ++k) {
+    k_start = k;
+    if (base_height <= z_face[k + 1]) break;
+  }
+  for (int k = 0; k < zsize; ++k) {
+    k_end = k + 1;
+    if (height_eff < z[k + 1]) break;
+  }
+
+  for (int j = j_start; j <= j_end; ++j) {
+    float y_cent = (j + 0.5) * dy;
+    for (int i = i_start; i <= i_end; ++i) {
+      float x_cent = (i + 0.5) * dx;
+      unsigned vert_id = 0;
+      unsigned start_poly = 0;
+      int num_crossing = 0;
+      while (vert_id < 5 - 1) {
+        if ((yv[vert_id] <= y_cent && yv[vert_id + 1] > y_cent)
+            || (yv[vert_id] > y_cent && yv[vert_id + 1] <= y_cent)) {
+          float ray_intersect = (y_cent - yv[vert_id]) / (yv[vert_id + 1] - yv[vert_id]);
+          if (x_cent < (xv[vert_id] + ray_intersect * (xv[vert_id + 1] - xv[vert_id]))) {
+            num_crossing += 1;
+          }
+        }
+        vert_id += 1;
+        if (xv[vert_id] == xv[start_poly] && yv[vert_id] == yv[start_poly]) {
+          vert_id += 1;
+          start_poly = vert_id;
+        }
+      }
+      if ((num_crossing % 2) != 0) {
+        if (i < 0 || j < 0 || i >= nx - 1 || j >= ny - 1) continue; /* outside: never in configs */
+        for (int k = k_start; k < k_end; ++k) {
+          long c = CELL(i, j, k);
+          if (icellflag[c] != 7) icellflag[c] = 0;
+        }
+      }
+    }
+  }
+}
+
+static inline int solid(int flag) { return flag == 0 || flag == 2; }
+
+/* Wall::defineWalls (Wall.cpp:44-349) for stair-step geometry (icellflag in
+ * {0,1,2}; the cut-cell flags 7/8 and their branches are out of scope):
+ * clamp (:48-91), wall rules (:128-163, repeated at :288-348), and the
+ * all-zero reset of solid cells (:251-286, a no-op for stair-step cells whose
+ * coefficients are never zeroed).  The east and north lookups use cellAdd and
+ * so wrap into the next row / layer at i = nx-2 / j = ny-2 (quirk A11). */
+void oracle_define_walls(int nx, int ny, int nz, const int *icellflag, float *e, float *f,
+                         float *g, float *h, float *m, float *n)
+{
+  const long sy = nx - 1, sz = (long)(nx - 1) * (ny - 1);
+  for (int k = 1; k < nz - 1; k++)
+    for (int i = 0; i < nx - 1; i++)
+      for (int j = 0; j < ny - 1; j++) {
+        long c = CELL(i, j, k);
+        float *arr[6] = {e, f, g, h, m, n};
+        for (int a = 0; a < 6; ++a) {
+          if (arr[a][c] < 0.05) arr[a][c] = 0.0f;
+          if (arr[a][c] > 1.0) arr[a][c] = 1.0f;
+        }
+      }
+  for (int k = 1; k < nz - 2; k++)
+    for (int i = 0; i < nx - 1; i++)
+      for (int j = 0; j < ny - 1; j++) {
+        long c = CELL(i, j, k);
+        if (icellflag[c] != 0 && icellflag[c] != 2) {
+          if (solid(icellflag[c - sz])) n[c] = 0.0f;
+          if (solid(icellflag[c + sz])) m[c] = 0.0f;
+          if (solid(icellflag[c - 1]) && i > 0) f[c] = 0.0f;
+          if (solid(icellflag[c + 1])) e[c] = 0.0f;
+          if (solid(icellflag[c - sy]) && j > 0) h[c] = 0.0f;
+          if (solid(icellflag[c + sy])) g[c] = 0.0f;
+        }
+      }
+}
+
+/* Wall::solverCoefficients (Wall.cpp:827-846) for k in [1,nz-3]; dy is dx
+ * (quirk A2, getDomainSize at Domain.h:104). */
+void oracle_solver_coefficients(int nx, int ny, int nz, float dx, const float *dz_array, float *e,
+                                float *f, float *g, float *h, float *m, float *n)
+{
+  const float dy = dx;
+  for (int k = 1; k < nz - 2; k++)
+    for (int j = 0; j < ny - 1; j++)
+      for (int i = 0; i < nx - 1; i++) {
+        long c = CELL(i, j, k);
+        e[c] = e[c] / (dx * dx);
+        f[c] = f[c] / (dx * dx);
+        g[c] = g[c] / (dy * dy);
+        h[c] = h[c] / (dy * dy);
+        m[c] = m[c] / (dz_array[k] * 0.5f * (dz_array[k] + dz_array[k + 1]));
+        n[c] = n[c] / (dz_array[k] * 0.5f * (dz_array[k] + dz_array[k - 1]));
+      }
+}
+
+/* Single-sensor vertical profile on flat terrain (terrain_face_id = 1,
+ * z_terrain = z_face[1] = 0), neutral stratification (1/L = 0), from
+ * WindProfilerSensorType::sensorsProfiles (WindProfilerSensorType.cpp:220,
+ * :290-295 uniform, :299-325 log, :328-333 power; the power-law exponent is
+ * the siteZ0 field, quirk A7).  kind: 0 uniform, 1 log, 2 power.  Writes
+ * u_prof/v_prof for k in [1, nz-2] (the layers the interpolation reads). */
+void oracle_profile(int nz, const float *z, int kind, float U_ref, float z_ref, float z0_or_p,
+                    float wind_dir, float *u_prof, float *v_prof)
+{
+  const float vk = 0.4f;
+  float theta = (float)((270.0 - wind_dir) * M_PI / 180.0);
+  float one_overL = 0.0f;
+  float z_terrain = 0.0f;
+  for (int k = 0; k < nz; ++k) u_prof[k] = v_prof[k] = 0.0f;
+  if (kind == 0) {
+    for (int k = 1; k < nz; ++k) {
+      u_prof[k] = cosf(theta) * U_ref;
+      v_prof[k] = sinf(theta) * U_ref;
+    }
+  } else if (kind == 1) {
+    float psi = (float)(4.7 * z_ref * one_overL);
+    float u_star = U_ref * vk / (logf((z_ref + z0_or_p) / z0_or_p) + psi);
+    for (int k = 1; k < nz - 1; ++k) {
+      psi = (float)(4.7 * (z[k] - z_terrain) * one_overL);
+      u_prof[k] = (cosf(theta) * u_star / vk) * (logf(((z[k] - z_terrain) + z0_or_p) / z0_or_p) + psi);
+      v_prof[k] = (sinf(theta) * u_star / vk) * (logf(((z[k] - z_terrain) + z0_or_p) / z0_or_p) + psi);
+      if (fabsf(u_star) >= 2 * U_ref) {
+        u_prof[k] = (cosf(theta) * U_ref);
+        v_prof[k] = (sinf(theta) * U_ref);
+      }
+    }
+  } else {
+    for (int k = 1; k < nz - 1; ++k) {
+      u_prof[k] = cosf(theta) * U_ref * powf(((z[k] - z_terrain) / z_ref), z0_or_p);
+      v_prof[k] = sinf(theta) * U_ref * powf(((z[k] - z_terrain) / z_ref), z0_or_p);
+    }
+  }
+}
+
+/* WindProfilerSensorType::singleSensorInterpolation on flat terrain
+ * (WindProfilerSensorType.cpp:571-621): u0 = u_prof[k], v0 = v_prof[k],
+ * w0 = 0 at every face of layers k in [1, nz-2]; other faces stay 0
+ * (applyWindProfile resizes to 0, WINDSGeneralData.cpp:1204-1209). */
+void oracle_extrude_profile(int nx, int ny, int nz, const float *u_prof, const float *v_prof,
+                            float *u0, float *v0, float *w0)
+{
+  const long nface = (long)nx * ny * nz;
+  for (long q = 0; q < nface; ++q) u0[q] = v0[q] = w0[q] = 0.0f;
+  for (int k = 1; k < nz - 1; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        long fc = FACE(i, j, k);
+        u0[fc] = u_prof[k];
+        v0[fc] = v_prof[k];
+        w0[fc] = 0.0f;
+      }
+}
+
+/* Wall::setVelocityZero (Wall.cpp:805-824): k in [0,nz-2], j in [1,ny-2],
+ * i in [1,nx-2]; all six faces of a solid cell set to 0. */
+void oracle_set_velocity_zero(int nx, int ny, int nz, const int *icellflag, float *u0, float *v0,
+                              float *w0)
+{
+  for (int k = 0; k < nz - 1; k++)
+    for (int j = 1; j < ny - 1; j++)
+      for (int i = 1; i < nx - 1; i++) {
+        long c = CELL(i, j, k), fc = FACE(i, j, k);
+        if (icellflag[c] == 0 || icellflag[c] == 2) {
+          u0[fc] = 0.0f;
+          u0[fc + 1] = 0.0f;
+          v0[fc] = 0.0f;
+          v0[fc + nx] = 0.0f;
+          w0[fc] = 0.0f;
+          w0[fc + (long)nx * ny] = 0.0f;
+        }
+      }
+}
+
+int oracle_num_threads(void)
+{
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+void oracle_set_num_threads(int t)
+{
+#ifdef _OPENMP
+  omp_set_num_threads(t);
+#else
+  (void)t;
+#endif
+}

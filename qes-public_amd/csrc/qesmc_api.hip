@@ -1,0 +1,454 @@
+// C ABI of libqesmc: context, device buffers, the solve schedule.
+//
+// Schedule of one solve (qesmc_run), all on the context's stream:
+//   [encode]  wall-mask stencil encoding (once per geometry, or per run on request)
+//   [pack]    lambda -> colour-split layout                      (split path)
+//   R         divergence of u0                                    Solver_CPU_RB.cpp:56-80
+//   loop t    red pass, black pass (gated on err[t] > tol)        Solver_CPU_RB.cpp:95-184
+//   [unpack]  lambda -> x-fastest layout                          (split path)
+//   mirror, correction, solid zeroing                             Solver_CPU_RB.cpp:154-162,194-284
+// The host never waits inside the loop except to poll convergence one chunk
+// behind the device, so no iteration pays a device->host round trip
+// (the reference copies max_error back every iteration,
+// Solver_GPU_GlobalMemory.cu:327).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "qesmc_internal.h"
+
+hipError_t qesmc_launch_divergence(qesmc_ctx *x, bool split, int alpha1);
+hipError_t qesmc_launch_stencil_encode(qesmc_ctx *x);
+hipError_t qesmc_launch_split_coefs(qesmc_ctx *x);
+hipError_t qesmc_launch_pack(qesmc_ctx *x, bool to_split);
+hipError_t qesmc_launch_iteration(qesmc_ctx *x, int path, bool masked, int t, int itermax,
+                                  float omega, float tol);
+hipError_t qesmc_launch_finish(qesmc_ctx *x, bool mirror, int alpha1, int alpha2);
+hipError_t qesmc_launch_div_linf(qesmc_ctx *x, int physical);
+hipError_t qesmc_launch_set_u32(qesmc_ctx *x, unsigned *p, unsigned v);
+
+namespace {
+
+int map_hip(hipError_t e)
+{
+  switch (e) {
+  case hipSuccess: return QESMC_OK;
+  case hipErrorOutOfMemory: return QESMC_ERR_OOM;
+  case hipErrorNoDevice:
+  case hipErrorInvalidDevice: return QESMC_ERR_NO_DEVICE;
+  default: return QESMC_ERR_HIP;
+  }
+}
+
+#define HIPCHK(expr)                      \
+  do {                                    \
+    hipError_t _e = (expr);               \
+    if (_e != hipSuccess) return map_hip(_e); \
+  } while (0)
+
+template <typename T>
+hipError_t dalloc(T **p, long long count)
+{
+  *p = nullptr;
+  if (count <= 0) return hipSuccess;
+  return hipMalloc((void **)p, sizeof(T) * (size_t)count);
+}
+
+template <typename T>
+void dfree(T *&p)
+{
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
+}
+
+void free_geometry(qesmc_ctx *x)
+{
+  for (auto &p : x->d_coef) dfree(p);
+  dfree(x->d_flag);
+  dfree(x->d_dz);
+  dfree(x->d_u0);
+  dfree(x->d_v0);
+  dfree(x->d_w0);
+  dfree(x->d_u);
+  dfree(x->d_v);
+  dfree(x->d_w);
+  dfree(x->d_R);
+  dfree(x->d_lam);
+  for (int c = 0; c < 2; ++c) {
+    dfree(x->d_lc[c]);
+    dfree(x->d_Rc[c]);
+    dfree(x->d_mask[c]);
+    for (auto &p : x->d_cc[c]) dfree(p);
+  }
+  dfree(x->d_canon);
+  x->have_geom = x->have_inflow = false;
+  x->geom_checked = false;
+  x->d = qesmc::Dims{};
+}
+
+int alloc_geometry(qesmc_ctx *x, int nx, int ny, int nz)
+{
+  qesmc::Dims d{};
+  d.nx = nx;
+  d.ny = ny;
+  d.nz = nz;
+  d.NX = nx - 1;
+  d.NY = ny - 1;
+  d.NZ = nz - 1;
+  const int half = (d.NX + 1) / 2;
+  d.HP = (half + 15) / 16 * 16;  // 64-byte aligned half-rows
+  d.sy = d.NX;
+  d.sz = (long long)d.NX * d.NY;
+  d.fy = nx;
+  d.fz = (long long)nx * ny;
+  d.splane = (long long)d.NY * d.HP;
+  x->ncell = (long long)d.NX * d.NY * d.NZ;
+  x->nface = (long long)nx * ny * nz;
+  x->nsplit = d.splane * d.NZ;
+  x->d = d;
+  for (auto &p : x->d_coef) HIPCHK(dalloc(&p, x->ncell));
+  HIPCHK(dalloc(&x->d_flag, x->ncell));
+  HIPCHK(dalloc(&x->d_dz, nz - 1));
+  HIPCHK(dalloc(&x->d_u0, x->nface));
+  HIPCHK(dalloc(&x->d_v0, x->nface));
+  HIPCHK(dalloc(&x->d_w0, x->nface));
+  HIPCHK(dalloc(&x->d_u, x->nface));
+  HIPCHK(dalloc(&x->d_v, x->nface));
+  HIPCHK(dalloc(&x->d_w, x->nface));
+  HIPCHK(dalloc(&x->d_R, x->ncell));
+  HIPCHK(dalloc(&x->d_lam, x->ncell));
+  for (int c = 0; c < 2; ++c) {
+    HIPCHK(dalloc(&x->d_lc[c], x->nsplit));
+    HIPCHK(dalloc(&x->d_Rc[c], x->nsplit));
+    HIPCHK(dalloc(&x->d_mask[c], x->nsplit));
+  }
+  HIPCHK(dalloc(&x->d_canon, 6LL * d.NZ));
+  HIPCHK(hipMemsetAsync(x->d_lam, 0, sizeof(float) * x->ncell, x->stream));
+  return QESMC_OK;
+}
+
+// Build the wall-mask encoding from the resident natural coefficients and
+// decide whether the masked stencil reproduces them exactly.
+int encode_stencil(qesmc_ctx *x)
+{
+  HIPCHK(hipMemsetAsync(x->d_status, 0, sizeof(unsigned), x->stream));
+  HIPCHK(hipMemsetAsync(x->d_canon, 0, sizeof(unsigned) * 6 * x->d.NZ, x->stream));
+  HIPCHK(qesmc_launch_stencil_encode(x));
+  HIPCHK(hipMemcpyAsync(x->h_poll + 2, x->d_status, sizeof(unsigned), hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  const unsigned st = x->h_poll[2];
+  if (st & qesmc::ST_ZERO_DIAG) return QESMC_ERR_ZERO_DIAG;
+  x->compressible = !(st & qesmc::ST_INCOMPRESSIBLE);
+  if (!x->compressible) {
+    for (int c = 0; c < 2; ++c)
+      for (auto &p : x->d_cc[c])
+        if (!p) HIPCHK(dalloc(&p, x->nsplit));
+    HIPCHK(qesmc_launch_split_coefs(x));
+  }
+  x->geom_checked = true;
+  return QESMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *qesmc_status_string(int s)
+{
+  switch (s) {
+  case QESMC_OK: return "ok";
+  case QESMC_ERR_ARG: return "invalid argument";
+  case QESMC_ERR_NO_DEVICE: return "no HIP device";
+  case QESMC_ERR_OOM: return "device out of memory";
+  case QESMC_ERR_HIP: return "HIP runtime error";
+  case QESMC_ERR_ZERO_DIAG: return "zero stencil diagonal (e+f+g+h+m+n == 0 in an updated cell)";
+  case QESMC_ERR_NONFINITE: return "lambda became non-finite";
+  case QESMC_ERR_STATE: return "inputs not set";
+  case QESMC_ERR_UNSUPPORTED: return "unsupported option";
+  default: return "unknown status";
+  }
+}
+
+void qesmc_default_params(qesmc_params *p)
+{
+  if (!p) return;
+  p->omega = 1.78f;
+  p->tol = 1e-9f;
+  p->itermax = 500;
+  p->alpha1 = 1;
+  p->alpha2 = 1;
+  p->path = QESMC_PATH_AUTO;
+  p->check_every = 64;
+  p->flags = 0;
+}
+
+int qesmc_create(qesmc_ctx **out, int device)
+{
+  if (!out) return QESMC_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return QESMC_ERR_NO_DEVICE;
+  if (device < 0 || device >= n) return QESMC_ERR_NO_DEVICE;
+  HIPCHK(hipSetDevice(device));
+  qesmc_ctx *x = new (std::nothrow) qesmc_ctx();
+  if (!x) return QESMC_ERR_OOM;
+  x->device = device;
+  int rc = QESMC_OK;
+  auto fail = [&](hipError_t e) {
+    rc = map_hip(e);
+    return e != hipSuccess;
+  };
+  if (fail(hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking))) goto bad;
+  for (auto &ev : x->ev)
+    if (fail(hipEventCreate(&ev))) goto bad;
+  if (fail(hipHostMalloc((void **)&x->h_poll, 8 * sizeof(unsigned), hipHostMallocDefault))) goto bad;
+  if (fail(dalloc(&x->d_status, 1))) goto bad;
+  if (fail(dalloc(&x->d_norm, 1))) goto bad;
+  *out = x;
+  return QESMC_OK;
+bad:
+  qesmc_destroy(x);
+  return rc == QESMC_OK ? QESMC_ERR_HIP : rc;
+}
+
+int qesmc_destroy(qesmc_ctx *x)
+{
+  if (!x) return QESMC_OK;
+  (void)hipSetDevice(x->device);
+  if (x->stream) (void)hipStreamSynchronize(x->stream);
+  free_geometry(x);
+  dfree(x->d_err);
+  dfree(x->d_status);
+  dfree(x->d_norm);
+  if (x->h_poll) (void)hipHostFree(x->h_poll);
+  for (auto &ev : x->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (x->stream) (void)hipStreamDestroy(x->stream);
+  delete x;
+  return QESMC_OK;
+}
+
+int qesmc_set_geometry(qesmc_ctx *x, const qesmc_grid *grid, const qesmc_coeffs *cf)
+{
+  if (!x || !grid || !cf || !grid->dz_array) return QESMC_ERR_ARG;
+  if (!cf->e || !cf->f || !cf->g || !cf->h || !cf->m || !cf->n || !cf->icellflag) return QESMC_ERR_ARG;
+  if (grid->nx < 2 || grid->ny < 2 || grid->nz < 3) return QESMC_ERR_ARG;
+  if (!(grid->dx > 0.f) || !(grid->dy > 0.f)) return QESMC_ERR_ARG;
+  const long long ncell = (long long)(grid->nx - 1) * (grid->ny - 1) * (grid->nz - 1);
+  if (ncell > (1LL << 31) - 1) return QESMC_ERR_UNSUPPORTED;  // per-plane int math
+  HIPCHK(hipSetDevice(x->device));
+  if (x->d.nx != grid->nx || x->d.ny != grid->ny || x->d.nz != grid->nz || !x->d_lam) {
+    HIPCHK(hipStreamSynchronize(x->stream));
+    free_geometry(x);
+    int rc = alloc_geometry(x, grid->nx, grid->ny, grid->nz);
+    if (rc != QESMC_OK) {
+      free_geometry(x);
+      return rc;
+    }
+  }
+  x->dx = grid->dx;
+  x->dy = grid->dy;
+  x->dz_host.assign(grid->dz_array, grid->dz_array + (grid->nz - 1));
+  const size_t cb = sizeof(float) * (size_t)x->ncell;
+  const float *src[6] = {cf->e, cf->f, cf->g, cf->h, cf->m, cf->n};
+  for (int a = 0; a < 6; ++a) HIPCHK(hipMemcpyAsync(x->d_coef[a], src[a], cb, hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipMemcpyAsync(x->d_flag, cf->icellflag, sizeof(int) * (size_t)x->ncell, hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipMemcpyAsync(x->d_dz, grid->dz_array, sizeof(float) * (grid->nz - 1), hipMemcpyHostToDevice, x->stream));
+  x->have_geom = false;
+  int rc = encode_stencil(x);
+  if (rc != QESMC_OK) return rc;
+  x->have_geom = true;
+  return QESMC_OK;
+}
+
+int qesmc_set_inflow(qesmc_ctx *x, const float *u0, const float *v0, const float *w0)
+{
+  if (!x || !u0 || !v0 || !w0) return QESMC_ERR_ARG;
+  if (!x->d_u0) return QESMC_ERR_STATE;
+  HIPCHK(hipSetDevice(x->device));
+  const size_t fb = sizeof(float) * (size_t)x->nface;
+  HIPCHK(hipMemcpyAsync(x->d_u0, u0, fb, hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipMemcpyAsync(x->d_v0, v0, fb, hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipMemcpyAsync(x->d_w0, w0, fb, hipMemcpyHostToDevice, x->stream));
+  x->have_inflow = true;
+  return QESMC_OK;
+}
+
+int qesmc_set_lambda(qesmc_ctx *x, const float *lambda)
+{
+  if (!x) return QESMC_ERR_ARG;
+  if (!x->d_lam) return QESMC_ERR_STATE;
+  HIPCHK(hipSetDevice(x->device));
+  if (lambda)
+    HIPCHK(hipMemcpyAsync(x->d_lam, lambda, sizeof(float) * (size_t)x->ncell, hipMemcpyHostToDevice, x->stream));
+  else
+    HIPCHK(hipMemsetAsync(x->d_lam, 0, sizeof(float) * (size_t)x->ncell, x->stream));
+  return QESMC_OK;
+}
+
+int qesmc_run(qesmc_ctx *x, const qesmc_params *pp, qesmc_result *res)
+{
+  if (!x || !pp) return QESMC_ERR_ARG;
+  if (!x->have_geom || !x->have_inflow) return QESMC_ERR_STATE;
+  qesmc_params p = *pp;
+  if (p.itermax < 0) return QESMC_ERR_ARG;
+  if (p.check_every < 1) p.check_every = 64;
+  if (p.path != QESMC_PATH_AUTO && p.path != QESMC_PATH_PLAIN && p.path != QESMC_PATH_SPLIT)
+    return QESMC_ERR_ARG;
+  const int path = (p.path == QESMC_PATH_PLAIN) ? QESMC_PATH_PLAIN : QESMC_PATH_SPLIT;
+  HIPCHK(hipSetDevice(x->device));
+
+  if (p.flags & QESMC_RUN_REENCODE) {
+    int rc = encode_stencil(x);
+    if (rc != QESMC_OK) return rc;
+  }
+  const bool masked = x->compressible;
+
+  if (x->err_cap < p.itermax + 1) {
+    dfree(x->d_err);
+    x->err_cap = 0;
+    const int cap = p.itermax + 1 > 1024 ? p.itermax + 1 : 1024;
+    HIPCHK(dalloc(&x->d_err, cap));
+    x->err_cap = cap;
+  }
+
+  hipStream_t s = x->stream;
+  HIPCHK(hipEventRecord(x->ev[0], s));
+  HIPCHK(hipMemsetAsync(x->d_status, 0, sizeof(unsigned), s));
+  HIPCHK(hipMemsetAsync(x->d_err, 0, sizeof(unsigned) * (size_t)(p.itermax + 1), s));
+  HIPCHK(qesmc_launch_set_u32(x, x->d_err, 0x3f800000u));  // 1.0f
+  if (path == QESMC_PATH_SPLIT) HIPCHK(qesmc_launch_pack(x, true));
+  HIPCHK(qesmc_launch_divergence(x, path == QESMC_PATH_SPLIT, p.alpha1));
+  HIPCHK(hipEventRecord(x->ev[1], s));
+
+  const bool first_runs = p.itermax > 0 && (1.0f > p.tol);
+  int launched = 0;
+  if (first_runs) {
+    // Poll convergence one chunk behind the device: after chunk n is queued,
+    // wait for chunk n-1's flag (already computed or about to be).
+    hipEvent_t poll_ev[2] = {nullptr, nullptr};
+    HIPCHK(hipEventCreateWithFlags(&poll_ev[0], hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&poll_ev[1], hipEventDisableTiming));
+    int chunk = 0;
+    bool stop = false;
+    int rc = QESMC_OK;
+    for (int t = 0; t < p.itermax && !stop; ++t) {
+      hipError_t e = qesmc_launch_iteration(x, path, masked, t, p.itermax, p.omega, p.tol);
+      if (e != hipSuccess) {
+        rc = map_hip(e);
+        break;
+      }
+      ++launched;
+      if ((t + 1) % p.check_every == 0 && t + 1 < p.itermax) {
+        const int slot = chunk & 1;
+        if (chunk >= 2) {
+          // slot is being reused: make sure its previous copy finished
+          if ((e = hipEventSynchronize(poll_ev[slot])) != hipSuccess) { rc = map_hip(e); break; }
+        }
+        e = hipMemcpyAsync(x->h_poll + slot, x->d_err + (t + 1), sizeof(unsigned), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipEventRecord(poll_ev[slot], s);
+        if (e != hipSuccess) { rc = map_hip(e); break; }
+        if (chunk >= 1) {
+          const int prev = slot ^ 1;
+          if ((e = hipEventSynchronize(poll_ev[prev])) != hipSuccess) { rc = map_hip(e); break; }
+          if (!(*(volatile float *)(x->h_poll + prev) > p.tol)) stop = true;
+        }
+        ++chunk;
+      }
+    }
+    (void)hipStreamSynchronize(s);
+    (void)hipEventDestroy(poll_ev[0]);
+    (void)hipEventDestroy(poll_ev[1]);
+    if (rc != QESMC_OK) return rc;
+  }
+  HIPCHK(hipEventRecord(x->ev[2], s));
+  if (path == QESMC_PATH_SPLIT) HIPCHK(qesmc_launch_pack(x, false));
+  HIPCHK(qesmc_launch_finish(x, first_runs, p.alpha1, p.alpha2));
+  HIPCHK(hipEventRecord(x->ev[3], s));
+  HIPCHK(hipStreamSynchronize(s));
+
+  // iteration count: iteration t ran iff err[t] > tol (err[0] = 1.0f)
+  std::vector<unsigned> herr((size_t)launched + 1);
+  HIPCHK(hipMemcpy(herr.data(), x->d_err, sizeof(unsigned) * herr.size(), hipMemcpyDeviceToHost));
+  int count = 0;
+  while (count < launched) {
+    float ev;
+    std::memcpy(&ev, &herr[count], 4);
+    if (!(ev > p.tol)) break;
+    ++count;
+  }
+  float last;
+  std::memcpy(&last, &herr[count], 4);
+  unsigned st = 0;
+  HIPCHK(hipMemcpy(&st, x->d_status, sizeof(unsigned), hipMemcpyDeviceToHost));
+  const int rc = (st & qesmc::ST_NONFINITE) ? QESMC_ERR_NONFINITE : QESMC_OK;
+
+  if (res) {
+    float ms_total = 0.f, ms_sor = 0.f;
+    (void)hipEventElapsedTime(&ms_total, x->ev[0], x->ev[3]);
+    (void)hipEventElapsedTime(&ms_sor, x->ev[1], x->ev[2]);
+    res->iterations = count;
+    res->path = path;
+    res->compressed = (path == QESMC_PATH_SPLIT && masked) ? 1 : 0;
+    res->status = rc;
+    res->final_max_dlambda = last;
+    res->ms_total = ms_total;
+    res->ms_sor = ms_sor;
+    const double lups = (double)(x->d.nx - 3) * (x->d.ny - 3) * (x->d.nz - 3) * count;
+    res->glups = ms_sor > 0.f ? lups / (ms_sor * 1e6) : 0.0;
+  }
+  return rc;
+}
+
+int qesmc_get_velocity(qesmc_ctx *x, float *u, float *v, float *w)
+{
+  if (!x || !u || !v || !w) return QESMC_ERR_ARG;
+  if (!x->d_u) return QESMC_ERR_STATE;
+  HIPCHK(hipSetDevice(x->device));
+  const size_t fb = sizeof(float) * (size_t)x->nface;
+  HIPCHK(hipMemcpyAsync(u, x->d_u, fb, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipMemcpyAsync(v, x->d_v, fb, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipMemcpyAsync(w, x->d_w, fb, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return QESMC_OK;
+}
+
+int qesmc_get_lambda(qesmc_ctx *x, float *lambda)
+{
+  if (!x || !lambda) return QESMC_ERR_ARG;
+  if (!x->d_lam) return QESMC_ERR_STATE;
+  HIPCHK(hipSetDevice(x->device));
+  HIPCHK(hipMemcpyAsync(lambda, x->d_lam, sizeof(float) * (size_t)x->ncell, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return QESMC_OK;
+}
+
+int qesmc_divergence_linf(qesmc_ctx *x, int physical, double *out)
+{
+  if (!x || !out) return QESMC_ERR_ARG;
+  if (!x->have_geom || !x->d_u) return QESMC_ERR_STATE;
+  HIPCHK(hipSetDevice(x->device));
+  HIPCHK(qesmc_launch_div_linf(x, physical));
+  HIPCHK(hipMemcpyAsync(out, x->d_norm, sizeof(double), hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return QESMC_OK;
+}
+
+int qesmc_solve(qesmc_ctx *x, const qesmc_grid *grid, const qesmc_coeffs *cf, qesmc_fields *fl,
+                float *lambda_inout, const qesmc_params *p, qesmc_result *res)
+{
+  if (!x || !grid || !cf || !fl || !p) return QESMC_ERR_ARG;
+  if (!fl->u0 || !fl->v0 || !fl->w0 || !fl->u || !fl->v || !fl->w) return QESMC_ERR_ARG;
+  int rc = qesmc_set_geometry(x, grid, cf);
+  if (rc != QESMC_OK) return rc;
+  if ((rc = qesmc_set_inflow(x, fl->u0, fl->v0, fl->w0)) != QESMC_OK) return rc;
+  if ((rc = qesmc_set_lambda(x, lambda_inout)) != QESMC_OK) return rc;
+  const int run_rc = qesmc_run(x, p, res);
+  if (run_rc != QESMC_OK && run_rc != QESMC_ERR_NONFINITE) return run_rc;
+  if ((rc = qesmc_get_velocity(x, fl->u, fl->v, fl->w)) != QESMC_OK) return rc;
+  if (lambda_inout && (rc = qesmc_get_lambda(x, lambda_inout)) != QESMC_OK) return rc;
+  return run_rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,70 @@
+// Internal declarations shared by the HIP translation units of libqesmc.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "qesmc.h"
+
+namespace qesmc {
+
+// Status bits raised by kernels into ctx->d_status.
+enum : unsigned {
+  ST_NONFINITE = 1u,      // a relaxed lambda was NaN or Inf
+  ST_INCOMPRESSIBLE = 2u, // a coefficient is neither 0 nor its layer's wall value
+  ST_ZERO_DIAG = 4u,      // an updated cell has e+f+g+h+m+n == 0
+};
+
+// Grid constants passed by value to every kernel.
+//   nx, ny, nz : staggered sizes (qes::Domain)
+//   NX, NY, NZ : cell counts per axis = nx-1, ny-1, nz-1
+//   HP         : padded length of one colour's half-row in the split layout
+struct Dims {
+  int nx, ny, nz;
+  int NX, NY, NZ;
+  int HP;
+  long long sy, sz;    // cell strides (NX, NX*NY)
+  long long fy, fz;    // face strides (nx, nx*ny)
+  long long splane;    // split-layout plane stride NY*HP
+};
+
+}  // namespace qesmc
+
+struct qesmc_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {};
+
+  qesmc::Dims d{};
+  float dx = 0.f, dy = 0.f;
+  long long ncell = 0, nface = 0, nsplit = 0;
+  std::vector<float> dz_host;
+  bool have_geom = false, have_inflow = false;
+  bool geom_checked = false;   // stencil encoding built for the current geometry
+  bool compressible = false;
+
+  // natural (x-fastest) device arrays
+  float *d_coef[6] = {};
+  int *d_flag = nullptr;
+  float *d_dz = nullptr;
+  float *d_u0 = nullptr, *d_v0 = nullptr, *d_w0 = nullptr;
+  float *d_u = nullptr, *d_v = nullptr, *d_w = nullptr;
+  float *d_R = nullptr;
+  float *d_lam = nullptr;
+
+  // colour-split arrays (index = colour)
+  float *d_lc[2] = {};
+  float *d_Rc[2] = {};
+  unsigned char *d_mask[2] = {};
+  float *d_cc[2][6] = {};     // full coefficients, split (only if not compressible)
+  unsigned *d_canon = nullptr; // 6*NZ wall values per layer (float bits)
+
+  // convergence slots: err[0] = 1.0f sentinel, err[t+1] = max|dlambda| of iteration t
+  unsigned *d_err = nullptr;
+  int err_cap = 0;
+  unsigned *d_status = nullptr;
+  double *d_norm = nullptr;
+  unsigned *h_poll = nullptr;  // pinned host slots for convergence polling
+};

@@ -1,0 +1,328 @@
+"""Python binding of libqesmc (include/qesmc.h) over ctypes.
+
+The product is the C ABI and its HIP kernels; this module only marshals numpy
+arrays into it.  It never falls back to a CPU implementation: if libqesmc.so
+is missing, or no HIP device is present, the calls raise.
+
+Layouts follow QES-Winds (src/qes/Domain.h:143-175): nx = ncx+1, ny = ncy+1,
+nz = ncz+2; cell arrays (nx-1)(ny-1)(nz-1), face arrays nx*ny*nz, x fastest.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libqesmc.so")
+
+OK = 0
+ERR_ARG = -1
+ERR_NO_DEVICE = -2
+ERR_OOM = -3
+ERR_HIP = -4
+ERR_ZERO_DIAG = -5
+ERR_NONFINITE = -6
+ERR_STATE = -7
+ERR_UNSUPPORTED = -8
+
+PATH_AUTO = 0
+PATH_PLAIN = 1
+PATH_SPLIT = 2
+RUN_REENCODE = 1
+
+# every symbol include/qesmc.h declares
+EXPORTS = (
+    "qesmc_create", "qesmc_destroy", "qesmc_default_params", "qesmc_status_string",
+    "qesmc_solve", "qesmc_set_geometry", "qesmc_set_inflow", "qesmc_set_lambda", "qesmc_run",
+    "qesmc_get_velocity", "qesmc_get_lambda", "qesmc_divergence_linf", "qesmc_city_build",
+    "qesmc_city_random_buildings",
+)
+
+
+class QesmcError(RuntimeError):
+    def __init__(self, code: int, where: str):
+        self.code = code
+        super().__init__(f"{where}: {status_string(code)} ({code})")
+
+
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int)
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("dx", C.c_float),
+                ("dy", C.c_float), ("dz_array", _fp)]
+
+
+class Coeffs(C.Structure):
+    _fields_ = [("e", _fp), ("f", _fp), ("g", _fp), ("h", _fp), ("m", _fp), ("n", _fp),
+                ("icellflag", _ip)]
+
+
+class Fields(C.Structure):
+    _fields_ = [("u0", _fp), ("v0", _fp), ("w0", _fp), ("u", _fp), ("v", _fp), ("w", _fp)]
+
+
+class Params(C.Structure):
+    _fields_ = [("omega", C.c_float), ("tol", C.c_float), ("itermax", C.c_int),
+                ("alpha1", C.c_int), ("alpha2", C.c_int), ("path", C.c_int),
+                ("check_every", C.c_int), ("flags", C.c_int)]
+
+
+class Result(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("path", C.c_int), ("compressed", C.c_int),
+                ("status", C.c_int), ("final_max_dlambda", C.c_double), ("ms_total", C.c_double),
+                ("ms_sor", C.c_double), ("glups", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Building(C.Structure):
+    _fields_ = [("x_start", C.c_float), ("y_start", C.c_float), ("length", C.c_float),
+                ("width", C.c_float), ("height", C.c_float), ("base_height", C.c_float),
+                ("rotation_deg", C.c_float)]
+
+
+class Inflow(C.Structure):
+    _fields_ = [("kind", C.c_int), ("U_ref", C.c_float), ("z_ref", C.c_float),
+                ("z0_or_p", C.c_float), ("wind_dir_deg", C.c_float)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libqesmc.so (built by `make -C qes-public_amd/csrc`)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libqesmc.so not built: {LIB_PATH} (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    L.qesmc_create.argtypes = [C.POINTER(P), C.c_int]
+    L.qesmc_destroy.argtypes = [P]
+    L.qesmc_default_params.argtypes = [C.POINTER(Params)]
+    L.qesmc_default_params.restype = None
+    L.qesmc_status_string.argtypes = [C.c_int]
+    L.qesmc_status_string.restype = C.c_char_p
+    L.qesmc_solve.argtypes = [P, C.POINTER(Grid), C.POINTER(Coeffs), C.POINTER(Fields), _fp,
+                              C.POINTER(Params), C.POINTER(Result)]
+    L.qesmc_set_geometry.argtypes = [P, C.POINTER(Grid), C.POINTER(Coeffs)]
+    L.qesmc_set_inflow.argtypes = [P, _fp, _fp, _fp]
+    L.qesmc_set_lambda.argtypes = [P, _fp]
+    L.qesmc_run.argtypes = [P, C.POINTER(Params), C.POINTER(Result)]
+    L.qesmc_get_velocity.argtypes = [P, _fp, _fp, _fp]
+    L.qesmc_get_lambda.argtypes = [P, _fp]
+    L.qesmc_divergence_linf.argtypes = [P, C.c_int, C.POINTER(C.c_double)]
+    L.qesmc_city_build.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, _fp, C.c_int,
+                                   C.POINTER(Building), C.POINTER(Inflow)] + [_fp] * 6 + [_ip] + [_fp] * 3
+    L.qesmc_city_random_buildings.argtypes = [C.c_ulonglong, C.c_int, C.c_int, C.c_float, C.c_int,
+                                              C.c_float, C.c_float, C.c_float, C.c_float, C.c_int,
+                                              C.c_float, C.c_int, C.POINTER(Building)]
+    for name in EXPORTS:
+        fn = getattr(L, name)
+        if name not in ("qesmc_default_params", "qesmc_status_string"):
+            fn.restype = C.c_int
+    _lib = L
+    return L
+
+
+def status_string(code: int) -> str:
+    return lib().qesmc_status_string(int(code)).decode()
+
+
+def _f(a):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_fp)
+
+
+def _i(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_ip)
+
+
+def _check(rc, where):
+    if rc != OK:
+        raise QesmcError(rc, where)
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    lib().qesmc_default_params(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class City:
+    """The solver's inputs (WINDSGeneralData arrays + Domain sizes)."""
+
+    def __init__(self, nx, ny, nz, dx, dy, dz_array, e, f, g, h, m, n, icellflag, u0, v0, w0):
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.dx, self.dy = float(dx), float(dy)
+        c = np.ascontiguousarray
+        self.dz_array = c(dz_array, np.float32)
+        self.e, self.f, self.g, self.h, self.m, self.n = (c(a, np.float32) for a in (e, f, g, h, m, n))
+        self.icellflag = c(icellflag, np.int32)
+        self.u0, self.v0, self.w0 = (c(a, np.float32) for a in (u0, v0, w0))
+
+    @property
+    def ncell(self):
+        return (self.nx - 1) * (self.ny - 1) * (self.nz - 1)
+
+    @property
+    def nface(self):
+        return self.nx * self.ny * self.nz
+
+    def interior_lups(self):
+        return (self.nx - 3) * (self.ny - 3) * (self.nz - 3)
+
+    def arrays(self):
+        return dict(nx=self.nx, ny=self.ny, nz=self.nz, dx=self.dx, dy=self.dy,
+                    dz_array=self.dz_array, e=self.e, f=self.f, g=self.g, h=self.h, m=self.m,
+                    n=self.n, icellflag=self.icellflag, u0=self.u0, v0=self.v0, w0=self.w0)
+
+
+def city_build(ncx, ncy, dx, dz_array, buildings=(), inflow=(1, 5.0, 10.0, 0.1, 270.0)) -> City:
+    """Flat stair-step city through qesmc_city_build."""
+    dz_array = np.ascontiguousarray(dz_array, np.float32)
+    nx, ny, nz = ncx + 1, ncy + 1, dz_array.size + 1
+    ncell, nface = (nx - 1) * (ny - 1) * (nz - 1), nx * ny * nz
+    bl = list(buildings)
+    barr = (Building * max(1, len(bl)))()
+    for q, b in enumerate(bl):
+        vals = (list(b) + [0.0, 0.0])[:7] if not isinstance(b, Building) else None
+        barr[q] = b if vals is None else Building(*[float(v) for v in vals])
+    inf = Inflow(int(inflow[0]), *[float(v) for v in inflow[1:5]])
+    co = [np.empty(ncell, np.float32) for _ in range(6)]
+    flag = np.empty(ncell, np.int32)
+    u0, v0, w0 = (np.empty(nface, np.float32) for _ in range(3))
+    rc = lib().qesmc_city_build(nx, ny, nz, float(dx), _f(dz_array), len(bl), barr, C.byref(inf),
+                                *[_f(a) for a in co], _i(flag), _f(u0), _f(v0), _f(w0))
+    _check(rc, "qesmc_city_build")
+    return City(nx, ny, nz, dx, dx, dz_array, *co, flag, u0, v0, w0)
+
+
+def random_buildings(seed, ncx, ncy, dx, count, lmin=10.0, lmax=40.0, hmin=10.0, hmax=60.0,
+                     lognormal=False, hcap=120.0, margin=2):
+    out = (Building * max(1, count))()
+    n = lib().qesmc_city_random_buildings(int(seed), ncx + 1, ncy + 1, float(dx), int(count),
+                                          lmin, lmax, hmin, hmax, 1 if lognormal else 0, hcap,
+                                          int(margin), out)
+    if n < 0:
+        raise QesmcError(n, "qesmc_city_random_buildings")
+    return [(b.x_start, b.y_start, b.length, b.width, b.height, b.base_height, b.rotation_deg)
+            for b in out[:n]]
+
+
+def dz_uniform(ncz, dz):
+    return np.full(ncz + 1, dz, np.float32)
+
+
+def dz_geometric(ncz, dz0, ratio):
+    """ncz layers growing geometrically from dz0, ghost dz_array[0] = dz_array[1]."""
+    cells = (dz0 * np.float64(ratio) ** np.arange(ncz)).astype(np.float32)
+    out = np.empty(ncz + 1, np.float32)
+    out[1:] = cells
+    out[0] = out[1]
+    return out
+
+
+class Context:
+    """One libqesmc context (device buffers + a HIP stream)."""
+
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        _check(lib().qesmc_create(C.byref(self._h), int(device)), "qesmc_create")
+        self.city = None
+
+    def close(self):
+        if self._h:
+            lib().qesmc_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_geometry(self, city: City):
+        g = Grid(city.nx, city.ny, city.nz, city.dx, city.dy, _f(city.dz_array))
+        co = Coeffs(*[_f(a) for a in (city.e, city.f, city.g, city.h, city.m, city.n)], _i(city.icellflag))
+        _check(lib().qesmc_set_geometry(self._h, C.byref(g), C.byref(co)), "qesmc_set_geometry")
+        self.city = city
+
+    def set_inflow(self, u0, v0, w0):
+        u0, v0, w0 = (np.ascontiguousarray(a, np.float32) for a in (u0, v0, w0))
+        _check(lib().qesmc_set_inflow(self._h, _f(u0), _f(v0), _f(w0)), "qesmc_set_inflow")
+
+    def set_lambda(self, lam=None):
+        if lam is None:
+            _check(lib().qesmc_set_lambda(self._h, None), "qesmc_set_lambda")
+        else:
+            lam = np.ascontiguousarray(lam, np.float32)
+            _check(lib().qesmc_set_lambda(self._h, _f(lam)), "qesmc_set_lambda")
+
+    def run(self, params: Params | None = None, allow_nonfinite=False, **kw) -> dict:
+        p = params if params is not None else default_params(**kw)
+        r = Result()
+        rc = lib().qesmc_run(self._h, C.byref(p), C.byref(r))
+        if rc != OK and not (allow_nonfinite and rc == ERR_NONFINITE):
+            raise QesmcError(rc, "qesmc_run")
+        return r.as_dict()
+
+    def get_velocity(self):
+        n = self.city.nface
+        u, v, w = (np.empty(n, np.float32) for _ in range(3))
+        _check(lib().qesmc_get_velocity(self._h, _f(u), _f(v), _f(w)), "qesmc_get_velocity")
+        return u, v, w
+
+    def get_lambda(self):
+        lam = np.empty(self.city.ncell, np.float32)
+        _check(lib().qesmc_get_lambda(self._h, _f(lam)), "qesmc_get_lambda")
+        return lam
+
+    def divergence_linf(self, physical=False) -> float:
+        out = C.c_double()
+        _check(lib().qesmc_divergence_linf(self._h, 1 if physical else 0, C.byref(out)),
+               "qesmc_divergence_linf")
+        return out.value
+
+    def solve(self, city: City, lam=None, params: Params | None = None, **kw):
+        """One-call drop-in (qesmc_solve): returns (u, v, w, lambda, result)."""
+        p = params if params is not None else default_params(**kw)
+        g = Grid(city.nx, city.ny, city.nz, city.dx, city.dy, _f(city.dz_array))
+        co = Coeffs(*[_f(a) for a in (city.e, city.f, city.g, city.h, city.m, city.n)], _i(city.icellflag))
+        u, v, w = (np.empty(city.nface, np.float32) for _ in range(3))
+        fl = Fields(_f(city.u0), _f(city.v0), _f(city.w0), _f(u), _f(v), _f(w))
+        lam_io = np.zeros(city.ncell, np.float32) if lam is None else np.array(lam, np.float32, copy=True)
+        r = Result()
+        rc = lib().qesmc_solve(self._h, C.byref(g), C.byref(co), C.byref(fl), _f(lam_io), C.byref(p),
+                               C.byref(r))
+        _check(rc, "qesmc_solve")
+        self.city = city
+        return u, v, w, lam_io, r.as_dict()
+
+
+def gpu_available() -> bool:
+    """True when a HIP device can be opened through the library."""
+    try:
+        h = C.c_void_p()
+        rc = lib().qesmc_create(C.byref(h), 0)
+        if rc == OK:
+            lib().qesmc_destroy(h)
+            return True
+        return False
+    except Exception:
+        return False
