@@ -95,7 +95,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--itermax", type=int, default=500)
     ap.add_argument("--path", choices=["auto", "plain", "split"], default="auto")
+    ap.add_argument("--tile-k", type=int, default=0, help="k planes per SOR work-group (0: library choice)")
+    ap.add_argument("--tile-v", type=int, default=0, help="split path: half-cells per lane (0: library choice)")
+    ap.add_argument("--wg-per-cu", type=int, default=0, help="split path: work-groups per CU (0: library choice)")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline sampling (0: skip)")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="only run 5 plain device copies of this many MiB (HBM counter calibration under rocprofv3)")
     a = ap.parse_args()
 
     if a.gpus != 1:
@@ -103,9 +108,24 @@ def main():
               "multi-GPU sharding is described in DESIGN.md but not built", file=sys.stderr)
         sys.exit(2)
 
+    if a.copy_calibration_mib:
+        # A copy of known size through torch's device copy kernel: its
+        # FETCH_SIZE / WRITE_SIZE per dispatch calibrate the counters.
+        import torch
+
+        n = a.copy_calibration_mib * (1 << 20) // 4
+        src = torch.arange(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n * 4}))
+        return
+
     city, desc = config2_city()
     path = {"auto": Q.PATH_AUTO, "plain": Q.PATH_PLAIN, "split": Q.PATH_SPLIT}[a.path]
-    params = Q.default_params(itermax=a.itermax, tol=1e-9, path=path, flags=Q.RUN_REENCODE)
+    params = Q.default_params(itermax=a.itermax, tol=1e-9, path=path, flags=Q.RUN_REENCODE, tile_k=a.tile_k,
+                              tile_v=a.tile_v, wg_per_cu=a.wg_per_cu)
     with Q.Context(0) as ctx:
         ctx.set_geometry(city)
         ctx.set_inflow(city.u0, city.v0, city.w0)
@@ -130,8 +150,11 @@ def main():
     bpl = bytes_per_lup(results[-1])
     launch_bytes = bpl * city.interior_lups() / 2
     achieved = launch_bytes / (launch_ms * 1e-3) / 1e9
-    kernel = {Q.PATH_PLAIN: "k_rb_plain", Q.PATH_SPLIT: "k_rb_split<%s>" % ("true" if results[-1]["compressed"]
-                                                                             else "false")}[results[-1]["path"]]
+    last = results[-1]
+    if last["path"] == Q.PATH_PLAIN:
+        kernel = "k_rb_plain"
+    else:
+        kernel = "k_sor_split<%s, %d>" % ("true" if last["compressed"] else "false", last["tile_v"])
     out = {
         "metric": METRIC,
         "value": round(glups, 3),
@@ -147,7 +170,8 @@ def main():
         "data": "synthetic (seeded city generator, inputs resident in HBM)",
         "config": {"workload": desc, "cells": city.ncell, "updates_per_iteration": city.interior_lups(),
                    "iterations": its, "omega": 1.78, "path": results[-1]["path"],
-                   "wall_mask_stencil": bool(results[-1]["compressed"])},
+                   "wall_mask_stencil": bool(last["compressed"]),
+                   "tile": [last["tile_k"], last["tile_v"], last["wg_per_cu"]]},
         "t_ref_ms": round(ms_step, 3),
         "sor_ms": round(ms_sor, 3),
         "div_linf_ref_operator": div_ref,

@@ -82,6 +82,11 @@ typedef struct qesmc_params {
   int check_every;  /* host polls convergence every this many iterations (>=1);
                        results never depend on it                             */
   int flags;        /* QESMC_RUN_* bits                                        */
+  int tile_k;       /* k planes marched by one SOR wave tile (0 = library's
+                       choice)                                                */
+  int tile_v;       /* split path: half-cells per lane, 1, 2 or 4 (0 = choice) */
+  int wg_per_cu;    /* split path: resident work-groups per CU (0 = choice)    */
+  /* The tile fields only change speed, never results. */
 } qesmc_params;
 
 /* qesmc_params.flags: rebuild the stencil encoding from the resident
@@ -97,6 +102,8 @@ typedef struct qesmc_result {
   double ms_total;           /* device time of R + SOR + correction (HIP events)  */
   double ms_sor;             /* device time of the SOR loop alone                 */
   double glups;              /* (nx-3)(ny-3)(nz-3) * iterations / ms_sor          */
+  int tile_k, tile_v, wg_per_cu; /* tiling that ran                               */
+  int reserved;
 } qesmc_result;
 
 typedef struct qesmc_ctx qesmc_ctx;
@@ -110,7 +117,7 @@ int qesmc_create(qesmc_ctx **out, int device);
 int qesmc_destroy(qesmc_ctx *ctx);
 
 /* Reference defaults: omega 1.78f, tol 1e-9f, itermax 500, alpha 1,
- * path AUTO, check_every 64, flags 0. */
+ * path AUTO, check_every 64, flags 0, tile_k 0. */
 void qesmc_default_params(qesmc_params *p);
 
 const char *qesmc_status_string(int status);

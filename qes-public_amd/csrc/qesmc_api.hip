@@ -135,9 +135,10 @@ int encode_stencil(qesmc_ctx *x)
   HIPCHK(hipMemsetAsync(x->d_status, 0, sizeof(unsigned), x->stream));
   HIPCHK(hipMemsetAsync(x->d_canon, 0, sizeof(unsigned) * 6 * x->d.NZ, x->stream));
   HIPCHK(qesmc_launch_stencil_encode(x));
-  HIPCHK(hipMemcpyAsync(x->h_poll + 2, x->d_status, sizeof(unsigned), hipMemcpyDeviceToHost, x->stream));
+  unsigned *hst = x->h_poll + 2 * qesmc::kErrSlots;
+  HIPCHK(hipMemcpyAsync(hst, x->d_status, sizeof(unsigned), hipMemcpyDeviceToHost, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
-  const unsigned st = x->h_poll[2];
+  const unsigned st = *hst;
   if (st & qesmc::ST_ZERO_DIAG) return QESMC_ERR_ZERO_DIAG;
   x->compressible = !(st & qesmc::ST_INCOMPRESSIBLE);
   if (!x->compressible) {
@@ -181,6 +182,9 @@ void qesmc_default_params(qesmc_params *p)
   p->path = QESMC_PATH_AUTO;
   p->check_every = 64;
   p->flags = 0;
+  p->tile_k = 0;
+  p->tile_v = 0;
+  p->wg_per_cu = 0;
 }
 
 int qesmc_create(qesmc_ctx **out, int device)
@@ -202,7 +206,12 @@ int qesmc_create(qesmc_ctx **out, int device)
   if (fail(hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking))) goto bad;
   for (auto &ev : x->ev)
     if (fail(hipEventCreate(&ev))) goto bad;
-  if (fail(hipHostMalloc((void **)&x->h_poll, 8 * sizeof(unsigned), hipHostMallocDefault))) goto bad;
+  {
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+      x->n_cu = ncu;
+  }
+  if (fail(hipHostMalloc((void **)&x->h_poll, (2 * qesmc::kErrSlots + 8) * sizeof(unsigned), hipHostMallocDefault))) goto bad;
   if (fail(dalloc(&x->d_status, 1))) goto bad;
   if (fail(dalloc(&x->d_norm, 1))) goto bad;
   *out = x;
@@ -304,32 +313,47 @@ int qesmc_run(qesmc_ctx *x, const qesmc_params *pp, qesmc_result *res)
     if (rc != QESMC_OK) return rc;
   }
   const bool masked = x->compressible;
+  if (p.tile_k < 0 || p.wg_per_cu < 0) return QESMC_ERR_ARG;
+  if (p.tile_v != 0 && p.tile_v != 1 && p.tile_v != 2 && p.tile_v != 4) return QESMC_ERR_ARG;
+  x->kplanes = p.tile_k > 0 ? p.tile_k : x->kplanes_auto;
+  x->tile_v = p.tile_v > 0 ? p.tile_v : x->tile_v_auto;
+  x->wg_per_cu = p.wg_per_cu > 0 ? p.wg_per_cu : x->wg_per_cu_auto;
 
-  if (x->err_cap < p.itermax + 1) {
+  const long long rows = (long long)p.itermax + 1;
+  if (x->err_cap < rows) {
     dfree(x->d_err);
     x->err_cap = 0;
-    const int cap = p.itermax + 1 > 1024 ? p.itermax + 1 : 1024;
-    HIPCHK(dalloc(&x->d_err, cap));
-    x->err_cap = cap;
+    const long long cap = rows > 1024 ? rows : 1024;
+    HIPCHK(dalloc(&x->d_err, cap * qesmc::kErrSlots));
+    x->err_cap = (int)cap;
   }
 
   hipStream_t s = x->stream;
   HIPCHK(hipEventRecord(x->ev[0], s));
   HIPCHK(hipMemsetAsync(x->d_status, 0, sizeof(unsigned), s));
-  HIPCHK(hipMemsetAsync(x->d_err, 0, sizeof(unsigned) * (size_t)(p.itermax + 1), s));
-  HIPCHK(qesmc_launch_set_u32(x, x->d_err, 0x3f800000u));  // 1.0f
+  HIPCHK(hipMemsetAsync(x->d_err, 0, sizeof(unsigned) * (size_t)(rows * qesmc::kErrSlots), s));
+  HIPCHK(qesmc_launch_set_u32(x, x->d_err, 0x3f800000u));  // row 0: 1.0f
   if (path == QESMC_PATH_SPLIT) HIPCHK(qesmc_launch_pack(x, true));
   HIPCHK(qesmc_launch_divergence(x, path == QESMC_PATH_SPLIT, p.alpha1));
   HIPCHK(hipEventRecord(x->ev[1], s));
+
+  auto row_max = [](const unsigned *row) {
+    float mx = 0.f;
+    for (int q = 0; q < qesmc::kErrSlots; ++q) {
+      float v;
+      std::memcpy(&v, row + q, 4);
+      mx = v > mx ? v : mx;
+    }
+    return mx;
+  };
 
   const bool first_runs = p.itermax > 0 && (1.0f > p.tol);
   int launched = 0;
   if (first_runs) {
     // Poll convergence one chunk behind the device: after chunk n is queued,
-    // wait for chunk n-1's flag (already computed or about to be).
-    hipEvent_t poll_ev[2] = {nullptr, nullptr};
-    HIPCHK(hipEventCreateWithFlags(&poll_ev[0], hipEventDisableTiming));
-    HIPCHK(hipEventCreateWithFlags(&poll_ev[1], hipEventDisableTiming));
+    // wait for chunk n-1's row (already computed or about to be).  The gate
+    // inside the kernels makes iterations after convergence no-ops, so the
+    // result never depends on when the host notices.
     int chunk = 0;
     bool stop = false;
     int rc = QESMC_OK;
@@ -342,25 +366,24 @@ int qesmc_run(qesmc_ctx *x, const qesmc_params *pp, qesmc_result *res)
       ++launched;
       if ((t + 1) % p.check_every == 0 && t + 1 < p.itermax) {
         const int slot = chunk & 1;
-        if (chunk >= 2) {
-          // slot is being reused: make sure its previous copy finished
-          if ((e = hipEventSynchronize(poll_ev[slot])) != hipSuccess) { rc = map_hip(e); break; }
-        }
-        e = hipMemcpyAsync(x->h_poll + slot, x->d_err + (t + 1), sizeof(unsigned), hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipEventRecord(poll_ev[slot], s);
+        unsigned *dst = x->h_poll + slot * qesmc::kErrSlots;
+        if (chunk >= 2 && (e = hipEventSynchronize(x->ev[4 + slot])) != hipSuccess) { rc = map_hip(e); break; }
+        e = hipMemcpyAsync(dst, x->d_err + (long long)(t + 1) * qesmc::kErrSlots,
+                           sizeof(unsigned) * qesmc::kErrSlots, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipEventRecord(x->ev[4 + slot], s);
         if (e != hipSuccess) { rc = map_hip(e); break; }
         if (chunk >= 1) {
           const int prev = slot ^ 1;
-          if ((e = hipEventSynchronize(poll_ev[prev])) != hipSuccess) { rc = map_hip(e); break; }
-          if (!(*(volatile float *)(x->h_poll + prev) > p.tol)) stop = true;
+          if ((e = hipEventSynchronize(x->ev[4 + prev])) != hipSuccess) { rc = map_hip(e); break; }
+          if (!(row_max(x->h_poll + prev * qesmc::kErrSlots) > p.tol)) stop = true;
         }
         ++chunk;
       }
     }
-    (void)hipStreamSynchronize(s);
-    (void)hipEventDestroy(poll_ev[0]);
-    (void)hipEventDestroy(poll_ev[1]);
-    if (rc != QESMC_OK) return rc;
+    if (rc != QESMC_OK) {
+      (void)hipStreamSynchronize(s);
+      return rc;
+    }
   }
   HIPCHK(hipEventRecord(x->ev[2], s));
   if (path == QESMC_PATH_SPLIT) HIPCHK(qesmc_launch_pack(x, false));
@@ -368,18 +391,12 @@ int qesmc_run(qesmc_ctx *x, const qesmc_params *pp, qesmc_result *res)
   HIPCHK(hipEventRecord(x->ev[3], s));
   HIPCHK(hipStreamSynchronize(s));
 
-  // iteration count: iteration t ran iff err[t] > tol (err[0] = 1.0f)
-  std::vector<unsigned> herr((size_t)launched + 1);
+  // iteration count: iteration t ran iff max(row t) > tol (row 0 = 1.0f)
+  std::vector<unsigned> herr(((size_t)launched + 1) * qesmc::kErrSlots);
   HIPCHK(hipMemcpy(herr.data(), x->d_err, sizeof(unsigned) * herr.size(), hipMemcpyDeviceToHost));
   int count = 0;
-  while (count < launched) {
-    float ev;
-    std::memcpy(&ev, &herr[count], 4);
-    if (!(ev > p.tol)) break;
-    ++count;
-  }
-  float last;
-  std::memcpy(&last, &herr[count], 4);
+  while (count < launched && row_max(&herr[(size_t)count * qesmc::kErrSlots]) > p.tol) ++count;
+  const float last = row_max(&herr[(size_t)count * qesmc::kErrSlots]);
   unsigned st = 0;
   HIPCHK(hipMemcpy(&st, x->d_status, sizeof(unsigned), hipMemcpyDeviceToHost));
   const int rc = (st & qesmc::ST_NONFINITE) ? QESMC_ERR_NONFINITE : QESMC_OK;
@@ -397,6 +414,10 @@ int qesmc_run(qesmc_ctx *x, const qesmc_params *pp, qesmc_result *res)
     res->ms_sor = ms_sor;
     const double lups = (double)(x->d.nx - 3) * (x->d.ny - 3) * (x->d.nz - 3) * count;
     res->glups = ms_sor > 0.f ? lups / (ms_sor * 1e6) : 0.0;
+    res->tile_k = x->kplanes;
+    res->tile_v = x->tile_v;
+    res->wg_per_cu = x->wg_per_cu;
+    res->reserved = 0;
   }
   return rc;
 }

@@ -17,6 +17,11 @@ enum : unsigned {
   ST_ZERO_DIAG = 4u,      // an updated cell has e+f+g+h+m+n == 0
 };
 
+// max|dlambda| of one iteration is accumulated into kErrSlots striped
+// words (block b -> slot b % kErrSlots): device-scope atomics from every
+// XCD on one address serialise at the memory side, so they are spread.
+constexpr int kErrSlots = 64;
+
 // Grid constants passed by value to every kernel.
 //   nx, ny, nz : staggered sizes (qes::Domain)
 //   NX, NY, NZ : cell counts per axis = nx-1, ny-1, nz-1
@@ -35,7 +40,7 @@ struct Dims {
 struct qesmc_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[4] = {};
+  hipEvent_t ev[6] = {};  // 0-3 timing marks, 4-5 convergence polling
 
   qesmc::Dims d{};
   float dx = 0.f, dy = 0.f;
@@ -61,10 +66,18 @@ struct qesmc_ctx {
   float *d_cc[2][6] = {};     // full coefficients, split (only if not compressible)
   unsigned *d_canon = nullptr; // 6*NZ wall values per layer (float bits)
 
-  // convergence slots: err[0] = 1.0f sentinel, err[t+1] = max|dlambda| of iteration t
+  // convergence slots, kErrSlots words per iteration: row 0 holds the 1.0f
+  // sentinel, row t+1 the striped max|dlambda| of iteration t
   unsigned *d_err = nullptr;
-  int err_cap = 0;
+  int err_cap = 0;             // rows allocated
   unsigned *d_status = nullptr;
   double *d_norm = nullptr;
-  unsigned *h_poll = nullptr;  // pinned host slots for convergence polling
+  unsigned *h_poll = nullptr;  // pinned host words: 2 polling rows + status
+  int kplanes = 8;             // k planes marched by one SOR wave tile (this run)
+  int kplanes_auto = 8;        // default when qesmc_params.tile_k == 0
+  int tile_v = 1;              // half-cells per lane of the split kernel (this run)
+  int tile_v_auto = 1;
+  int wg_per_cu = 8;           // resident work-groups per CU of the split kernel
+  int wg_per_cu_auto = 8;
+  int n_cu = 256;              // compute units of the device
 };

@@ -44,11 +44,24 @@ __device__ inline float stop_delta(float nv, float ov)
   return (dl > 0.f) ? dl : 0.f;
 }
 
-// Iteration gate: iteration t runs iff err[t] > tol, with err[0] = 1.0f, the
-// reference's initial max_error (Solver_CPU_RB.cpp:91,95).
+// Iteration gate: iteration t runs iff max(err row t) > tol, with row 0
+// holding 1.0f, the reference's initial max_error (Solver_CPU_RB.cpp:91,95).
+// Lane l of each wave reads slot l of the row (blockDim.x == 64).
 __device__ inline bool gate_open(const unsigned *err, int t, float tol)
 {
-  return __uint_as_float(err[t]) > tol;
+  float v = __uint_as_float(err[(long long)t * kErrSlots + threadIdx.x]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  return v > tol;
+}
+
+// Block's max|dlambda| into its striped slot of row t+1.
+__device__ inline void publish_delta(unsigned *err, int t, float bm)
+{
+  if (threadIdx.x == 0 && threadIdx.y == 0 && bm > 0.f) {
+    const unsigned b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    atomicMax(err + (long long)(t + 1) * kErrSlots + (b % kErrSlots), __float_as_uint(bm));
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -96,7 +109,8 @@ __global__ __launch_bounds__(256) void k_divergence(Dims d, float dx, float dy,
 // reader of lambda(i,j,0) is cell (i,j,1) itself, so it is written right
 // after that cell is relaxed.  max|dlambda| (:164-180) is formed in registers.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_rb_plain(Dims d, int colour, const float *__restrict__ e,
+__global__ __launch_bounds__(256) void k_rb_plain(Dims d, int colour, int kplanes,
+                                                  const float *__restrict__ e,
                                                   const float *__restrict__ f,
                                                   const float *__restrict__ g,
                                                   const float *__restrict__ h,
@@ -109,25 +123,28 @@ __global__ __launch_bounds__(256) void k_rb_plain(Dims d, int colour, const floa
 {
   if (t >= itermax || !gate_open(err, t, tol)) return;  // uniform per launch
   const int j = blockIdx.y * 4 + threadIdx.y + 1;
-  const int k = blockIdx.z + 1;
-  const int s = (colour + j + k) & 1;
-  const int i = 2 * (blockIdx.x * 64 + threadIdx.x) + s;
+  const int tq = blockIdx.x * 64 + threadIdx.x;
+  const int k0 = 1 + blockIdx.z * kplanes;
+  const int k1 = min(k0 + kplanes, d.NZ - 1);
   float dl = 0.f;
-  if (j <= d.NY - 2 && i >= 1 && i <= d.NX - 2) {
-    const long long c = i + j * d.sy + k * d.sz;
-    const float ec = e[c], fc = f[c], gc = g[c], hc = h[c], mc = m[c], nc = n[c];
-    const float old = lam[c];
-    const float nv = (omega / (ec + fc + gc + hc + mc + nc))
-                         * (ec * lam[c + 1] + fc * lam[c - 1] + gc * lam[c + d.sy]
-                            + hc * lam[c - d.sy] + mc * lam[c + d.sz] + nc * lam[c - d.sz] - R[c])
-                     + one_m_omega * old;
-    lam[c] = nv;
-    if (k == 1) lam[c - d.sz] = nv;
-    if (!isfinite(nv)) atomicOr(status, ST_NONFINITE);
-    dl = stop_delta(nv, old);
+  for (int k = k0; k < k1; ++k) {
+    const int s = (colour + j + k) & 1;
+    const int i = 2 * tq + s;
+    if (j <= d.NY - 2 && i >= 1 && i <= d.NX - 2) {
+      const long long c = i + j * d.sy + k * d.sz;
+      const float ec = e[c], fc = f[c], gc = g[c], hc = h[c], mc = m[c], nc = n[c];
+      const float old = lam[c];
+      const float nv = (omega / (ec + fc + gc + hc + mc + nc))
+                           * (ec * lam[c + 1] + fc * lam[c - 1] + gc * lam[c + d.sy]
+                              + hc * lam[c - d.sy] + mc * lam[c + d.sz] + nc * lam[c - d.sz] - R[c])
+                       + one_m_omega * old;
+      lam[c] = nv;
+      if (k == 1) lam[c - d.sz] = nv;
+      if (!isfinite(nv)) atomicOr(status, ST_NONFINITE);
+      dl = fmaxf(dl, stop_delta(nv, old));
+    }
   }
-  const float bm = block_max_64x4(dl);
-  if (threadIdx.x == 0 && threadIdx.y == 0 && bm > 0.f) atomicMax(err + t + 1, __float_as_uint(bm));
+  publish_delta(err, t, block_max_64x4(dl));
 }
 
 // ---------------------------------------------------------------------------
@@ -139,6 +156,8 @@ __global__ __launch_bounds__(256) void k_rb_plain(Dims d, int colour, const floa
 
 // Wall value of each direction on each layer: the largest coefficient over
 // the updated cells (stair-step coefficients are 0 or that value).
+// gridDim.x work-groups per plane (k = blockIdx.y + 1) stride over the
+// plane's updated cells; 6 atomics per work-group on per-plane addresses.
 __global__ __launch_bounds__(256) void k_canon_reduce(Dims d, const float *__restrict__ e,
                                                       const float *__restrict__ f,
                                                       const float *__restrict__ g,
@@ -147,22 +166,26 @@ __global__ __launch_bounds__(256) void k_canon_reduce(Dims d, const float *__res
                                                       const float *__restrict__ n,
                                                       unsigned *canon, unsigned *status)
 {
-  const int i = blockIdx.x * 64 + threadIdx.x + 1;
-  const int j = blockIdx.y * 4 + threadIdx.y + 1;
-  const int k = blockIdx.z + 1;
-  const bool in = (i <= d.NX - 2 && j <= d.NY - 2);
-  const long long c = i + (long long)j * d.sy + k * d.sz;
+  const int k = blockIdx.y + 1;
+  const int W = d.NX - 2;
+  const int nplane = W * (d.NY - 2);
   const float *arr[6] = {e, f, g, h, m, n};
+  float mx[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   unsigned bad = 0;
+  for (int idx = blockIdx.x * 256 + threadIdx.y * 64 + threadIdx.x; idx < nplane; idx += gridDim.x * 256) {
+    const int jr = idx / W;
+    const long long c = (idx - jr * W + 1) + (long long)(jr + 1) * d.sy + k * d.sz;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) {
+      const float v = arr[a][c];
+      // negative, NaN or Inf coefficients cannot be encoded as a wall bit
+      if (!(v >= 0.f) || isinf(v)) bad = 1;
+      else mx[a] = v > mx[a] ? v : mx[a];
+    }
+  }
 #pragma unroll
   for (int a = 0; a < 6; ++a) {
-    float v = in ? arr[a][c] : 0.f;
-    // negative, NaN or Inf coefficients cannot be encoded as a wall bit
-    if (!(v >= 0.f) || isinf(v)) {
-      bad = 1;
-      v = 0.f;
-    }
-    const float bm = block_max_64x4(v);
+    const float bm = block_max_64x4(mx[a]);
     if (threadIdx.x == 0 && threadIdx.y == 0 && bm > 0.f) atomicMax(canon + a * d.NZ + k, __float_as_uint(bm));
     __syncthreads();
   }
@@ -243,64 +266,6 @@ __global__ __launch_bounds__(256) void k_unpack(Dims d, float *__restrict__ lam,
   const long long c = i + (long long)j * d.sy + k * d.sz;
   const long long s = (long long)(k * d.NY + j) * d.HP + (i >> 1);
   lam[c] = (((i + j + k) & 1) ? l1 : l0)[s];
-}
-
-// One colour pass on the split layout, in place on `own`, reading `oth`.
-template <bool MASKED>
-__global__ __launch_bounds__(256) void k_rb_split(Dims d, int colour, float *__restrict__ own,
-                                                  float *oth, const float *__restrict__ Rc,
-                                                  const unsigned char *__restrict__ mask,
-                                                  const unsigned *__restrict__ canon,
-                                                  const float *__restrict__ ce,
-                                                  const float *__restrict__ cf,
-                                                  const float *__restrict__ cg,
-                                                  const float *__restrict__ ch,
-                                                  const float *__restrict__ cm,
-                                                  const float *__restrict__ cn, float omega,
-                                                  float one_m_omega, float tol, unsigned *err,
-                                                  int t, int itermax, unsigned *status)
-{
-  if (t >= itermax || !gate_open(err, t, tol)) return;
-  const int q = blockIdx.x * 64 + threadIdx.x;
-  const int j = blockIdx.y * 4 + threadIdx.y + 1;
-  const int k = blockIdx.z + 1;
-  const int s = (j + k + colour) & 1;
-  const int i = 2 * q + s;
-  float dl = 0.f;
-  if (j <= d.NY - 2 && i >= 1 && i <= d.NX - 2) {
-    const long long x = (long long)(k * d.NY + j) * d.HP + q;
-    float ec, fc, gc, hc, mc, nc;
-    if (MASKED) {
-      const unsigned mk = mask[x];
-      ec = (mk & 1u) ? __uint_as_float(canon[0 * d.NZ + k]) : 0.f;
-      fc = (mk & 2u) ? __uint_as_float(canon[1 * d.NZ + k]) : 0.f;
-      gc = (mk & 4u) ? __uint_as_float(canon[2 * d.NZ + k]) : 0.f;
-      hc = (mk & 8u) ? __uint_as_float(canon[3 * d.NZ + k]) : 0.f;
-      mc = (mk & 16u) ? __uint_as_float(canon[4 * d.NZ + k]) : 0.f;
-      nc = (mk & 32u) ? __uint_as_float(canon[5 * d.NZ + k]) : 0.f;
-    } else {
-      ec = ce[x];
-      fc = cf[x];
-      gc = cg[x];
-      hc = ch[x];
-      mc = cm[x];
-      nc = cn[x];
-    }
-    const float old = own[x];
-    const float xp = oth[x + s], xm = oth[x - 1 + s];
-    const float yp = oth[x + d.HP], ym = oth[x - d.HP];
-    const float zp = oth[x + d.splane], zm = oth[x - d.splane];
-    const float nv = (omega / (ec + fc + gc + hc + mc + nc))
-                         * (ec * xp + fc * xm + gc * yp + hc * ym + mc * zp + nc * zm - Rc[x])
-                     + one_m_omega * old;
-    own[x] = nv;
-    // mirror: cell (i,j,0) has the other colour and the same half-index
-    if (k == 1) oth[x - d.splane] = nv;
-    if (!isfinite(nv)) atomicOr(status, ST_NONFINITE);
-    dl = stop_delta(nv, old);
-  }
-  const float bm = block_max_64x4(dl);
-  if (threadIdx.x == 0 && threadIdx.y == 0 && bm > 0.f) atomicMax(err + t + 1, __float_as_uint(bm));
 }
 
 // ---------------------------------------------------------------------------
@@ -391,29 +356,31 @@ __global__ __launch_bounds__(256) void k_div_linf(Dims d, double dx, double dy,
                                                   const float *__restrict__ w, int physical,
                                                   unsigned long long *out)
 {
-  const int i = blockIdx.x * 64 + threadIdx.x + 1;
-  const int j = blockIdx.y * 4 + threadIdx.y + 1;
-  const int k = blockIdx.z + 1;
-  double dv = 0.0;
-  if (i <= d.NX - 2 && j <= d.NY - 2) {
+  const int k = blockIdx.y + 1;
+  const int W = d.NX - 2;
+  const int nplane = W * (d.NY - 2);
+  const double dzk = dz[k];
+  double mx = 0.0;
+  for (int idx = blockIdx.x * 256 + threadIdx.y * 64 + threadIdx.x; idx < nplane; idx += gridDim.x * 256) {
+    const int jr = idx / W;
+    const int i = idx - jr * W + 1, j = jr + 1;
     const long long c = i + (long long)j * d.sy + k * d.sz;
-    if (flag[c] == 1) {
-      const long long fc = i + (long long)j * d.fy + k * d.fz;
-      const double dzk = dz[k];
-      if (physical) {
-        dv = ((double)u[fc + 1] - u[fc]) / dx + ((double)v[fc + d.fy] - v[fc]) / dy
-             + ((double)w[fc + d.fz] - w[fc]) / dzk;
-      } else {
-        dv = (((double)e[c] * u[fc + 1] - (double)f[c] * u[fc]) * dx)
-             + (((double)g[c] * v[fc + d.fy] - (double)h[c] * v[fc]) * dy)
-             + ((double)m[c] * dzk * 0.5 * (dzk + (double)dz[k + 1]) * w[fc + d.fz]
-                - (double)n[c] * dzk * 0.5 * (dzk + (double)dz[k - 1]) * w[fc]);
-      }
-      dv = fabs(dv);
-      if (!(dv >= 0.0)) dv = 0.0;
+    if (flag[c] != 1) continue;
+    const long long fc = i + (long long)j * d.fy + k * d.fz;
+    double dv;
+    if (physical) {
+      dv = ((double)u[fc + 1] - u[fc]) / dx + ((double)v[fc + d.fy] - v[fc]) / dy
+           + ((double)w[fc + d.fz] - w[fc]) / dzk;
+    } else {
+      dv = (((double)e[c] * u[fc + 1] - (double)f[c] * u[fc]) * dx)
+           + (((double)g[c] * v[fc + d.fy] - (double)h[c] * v[fc]) * dy)
+           + ((double)m[c] * dzk * 0.5 * (dzk + (double)dz[k + 1]) * w[fc + d.fz]
+              - (double)n[c] * dzk * 0.5 * (dzk + (double)dz[k - 1]) * w[fc]);
     }
+    dv = fabs(dv);
+    if (dv > mx) mx = dv;  // NaN never wins
   }
-  const double bm = block_max_64x4_d(dv);
+  const double bm = block_max_64x4_d(mx);
   if (threadIdx.x == 0 && threadIdx.y == 0 && bm > 0.0)
     atomicMax(out, (unsigned long long)__double_as_longlong(bm));
 }
@@ -428,10 +395,14 @@ static inline dim3 grid_xy(long long nxthreads, long long nythreads, long long n
   return dim3((unsigned)((nxthreads + 63) / 64), (unsigned)((nythreads + 3) / 4), (unsigned)nzblocks);
 }
 static const dim3 kBlock(64, 4, 1);
+static const unsigned kPlaneBlocks = 8;  // work-groups per plane of the plane reductions
 
 }  // namespace qesmc
 
 using namespace qesmc;
+
+hipError_t qesmc_launch_split_iteration(qesmc_ctx *x, bool masked, int t, int itermax, float omega,
+                                        float tol);
 
 // Entry points used by the API translation unit.
 hipError_t qesmc_launch_divergence(qesmc_ctx *x, bool split, int alpha1)
@@ -456,7 +427,7 @@ hipError_t qesmc_launch_stencil_encode(qesmc_ctx *x)
   const Dims &d = x->d;
   if (d.NZ - 2 < 1 || d.NX < 3 || d.NY < 3) return hipSuccess;
   dim3 gr = grid_xy(d.NX - 2, d.NY - 2, d.NZ - 2);
-  hipLaunchKernelGGL(k_canon_reduce, gr, kBlock, 0, x->stream, d, x->d_coef[0], x->d_coef[1],
+  hipLaunchKernelGGL(k_canon_reduce, dim3(kPlaneBlocks, d.NZ - 2), kBlock, 0, x->stream, d, x->d_coef[0], x->d_coef[1],
                      x->d_coef[2], x->d_coef[3], x->d_coef[4], x->d_coef[5], x->d_canon, x->d_status);
   hipLaunchKernelGGL(k_build_mask, gr, kBlock, 0, x->stream, d, x->d_coef[0], x->d_coef[1],
                      x->d_coef[2], x->d_coef[3], x->d_coef[4], x->d_coef[5], x->d_canon,
@@ -491,27 +462,15 @@ hipError_t qesmc_launch_iteration(qesmc_ctx *x, int path, bool masked, int t, in
   const Dims &d = x->d;
   if (d.NZ - 2 < 1 || d.NY - 2 < 1) return hipSuccess;
   const float omo = 1.0f - omega;
+  const int kp = x->kplanes < 1 ? 1 : x->kplanes;
+  const dim3 gr = grid_xy((d.NX + 1) / 2, d.NY - 2, (d.NZ - 2 + kp - 1) / kp);
   if (path == QESMC_PATH_PLAIN) {
-    dim3 gr = grid_xy((d.NX + 1) / 2, d.NY - 2, d.NZ - 2);
     for (int colour = 0; colour < 2; ++colour)
-      hipLaunchKernelGGL(k_rb_plain, gr, kBlock, 0, x->stream, d, colour, x->d_coef[0],
+      hipLaunchKernelGGL(k_rb_plain, gr, kBlock, 0, x->stream, d, colour, kp, x->d_coef[0],
                          x->d_coef[1], x->d_coef[2], x->d_coef[3], x->d_coef[4], x->d_coef[5],
                          x->d_R, x->d_lam, omega, omo, tol, x->d_err, t, itermax, x->d_status);
   } else {
-    dim3 gr = grid_xy((d.NX + 1) / 2, d.NY - 2, d.NZ - 2);
-    for (int colour = 0; colour < 2; ++colour) {
-      float *own = x->d_lc[colour], *oth = x->d_lc[colour ^ 1];
-      const float *const *cc = x->d_cc[colour];
-      if (masked)
-        hipLaunchKernelGGL(k_rb_split<true>, gr, kBlock, 0, x->stream, d, colour, own, oth,
-                           x->d_Rc[colour], x->d_mask[colour], x->d_canon, nullptr, nullptr,
-                           nullptr, nullptr, nullptr, nullptr, omega, omo, tol, x->d_err, t,
-                           itermax, x->d_status);
-      else
-        hipLaunchKernelGGL(k_rb_split<false>, gr, kBlock, 0, x->stream, d, colour, own, oth,
-                           x->d_Rc[colour], nullptr, nullptr, cc[0], cc[1], cc[2], cc[3], cc[4],
-                           cc[5], omega, omo, tol, x->d_err, t, itermax, x->d_status);
-    }
+    return qesmc_launch_split_iteration(x, masked, t, itermax, omega, tol);
   }
   return hipGetLastError();
 }
@@ -538,7 +497,7 @@ hipError_t qesmc_launch_div_linf(qesmc_ctx *x, int physical)
   hipError_t st = hipMemsetAsync(x->d_norm, 0, sizeof(double), x->stream);
   if (st != hipSuccess) return st;
   if (d.NZ - 2 < 1 || d.NX < 3 || d.NY < 3) return hipSuccess;
-  hipLaunchKernelGGL(k_div_linf, grid_xy(d.NX - 2, d.NY - 2, d.NZ - 2), kBlock, 0, x->stream, d,
+  hipLaunchKernelGGL(k_div_linf, dim3(kPlaneBlocks, d.NZ - 2), kBlock, 0, x->stream, d,
                      (double)x->dx, (double)x->dy, x->d_dz, x->d_coef[0], x->d_coef[1],
                      x->d_coef[2], x->d_coef[3], x->d_coef[4], x->d_coef[5], x->d_flag, x->d_u,
                      x->d_v, x->d_w, physical, (unsigned long long *)x->d_norm);

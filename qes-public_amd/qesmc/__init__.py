@@ -68,13 +68,15 @@ class Fields(C.Structure):
 class Params(C.Structure):
     _fields_ = [("omega", C.c_float), ("tol", C.c_float), ("itermax", C.c_int),
                 ("alpha1", C.c_int), ("alpha2", C.c_int), ("path", C.c_int),
-                ("check_every", C.c_int), ("flags", C.c_int)]
+                ("check_every", C.c_int), ("flags", C.c_int), ("tile_k", C.c_int),
+                ("tile_v", C.c_int), ("wg_per_cu", C.c_int)]
 
 
 class Result(C.Structure):
     _fields_ = [("iterations", C.c_int), ("path", C.c_int), ("compressed", C.c_int),
                 ("status", C.c_int), ("final_max_dlambda", C.c_double), ("ms_total", C.c_double),
-                ("ms_sor", C.c_double), ("glups", C.c_double)]
+                ("ms_sor", C.c_double), ("glups", C.c_double), ("tile_k", C.c_int),
+                ("tile_v", C.c_int), ("wg_per_cu", C.c_int), ("reserved", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -220,3 +220,13 @@ def test_config2_full_size_reference_schedule(gpu_ctx, oracle, qesmc):
         assert_same(u, ou, "u")
         assert_same(v, ov, "v")
         assert_same(w, ow, "w")
+
+
+@pytest.mark.parametrize("tile", [(1, 1, 1), (3, 1, 2), (8, 2, 8), (16, 4, 4), (5, 4, 16), (64, 2, 1)])
+def test_tile_parameters_never_change_results(gpu_ctx, oracle, qesmc, tile):
+    """(tile_k, tile_v, wg_per_cu) reshape the work decomposition only."""
+    tk, tv, wg = tile
+    p = random_city(oracle, qesmc, 77, 150, 90, 21)
+    res = check_solve(gpu_ctx, oracle, p, SPLIT, itermax=40, tile_k=tk, tile_v=tv, wg_per_cu=wg)
+    assert res["compressed"] == 1
+    check_solve(gpu_ctx, oracle, p, PLAIN, itermax=12, tile_k=tk)
