@@ -1,0 +1,81 @@
+/*
+ * Solver_HIP.h -- QES-Winds Solver subclass backed by libqesmc (MI355X).
+ *
+ * Drop-in beside Solver_CPU_RB / Solver_GPU_GlobalMemory / _SharedMemory:
+ * same base class (src/winds/Solver.h:61-120), same constructor signature as
+ * Solver_GPU_GlobalMemory (Solver_GPU_GlobalMemory.h:70), same
+ * solve(WINDSGeneralData*, const int&) contract (Solver.h:112): reads
+ * WGD->{e,f,g,h,m,n,icellflag,u0,v0,w0} and domain.dz_array, writes
+ * WGD->{u,v,w}.  Registered in SolverFactory as solveType 5 (see the
+ * integration notes shipped with the library).
+ *
+ * Header-only; compiles against the reference's Solver.h inside a QES tree,
+ * or against the minimal re-declaration in host/qes_compat for the library's
+ * own tests.  Links libqesmc.so (C ABI, include/qesmc.h).
+ *
+ * Differences from the reference GPU solvers, by design:
+ *  - device buffers persist across solves (one context per solver object);
+ *  - lambda is copied back into Solver::lambda, so a solver that is not
+ *    reset between time steps warm-starts, exactly like the CPU solvers
+ *    (qes/qesMain.cpp:136-150);
+ *  - errors come back as status codes and are turned into QESout::error here,
+ *    which keeps the reference's "report and exit" behaviour.
+ */
+#pragma once
+
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "Solver.h"
+#include "qesmc.h"
+#include "util/QESout.h"
+
+class Solver_HIP : public Solver
+{
+public:
+  Solver_HIP(qes::Domain domain_in, const float &tolerance, int device = 0)
+    : Solver(std::move(domain_in), tolerance)
+  {
+    const int rc = qesmc_create(&ctx_, device);
+    if (rc != QESMC_OK) QESout::error(std::string("[Solver_HIP] ") + qesmc_status_string(rc));
+  }
+
+  ~Solver_HIP() override { qesmc_destroy(ctx_); }
+
+  Solver_HIP(const Solver_HIP &) = delete;
+  Solver_HIP &operator=(const Solver_HIP &) = delete;
+
+  /** Iterations, last max|dlambda| and device timings of the last solve. */
+  const qesmc_result &lastResult() const { return last_; }
+
+  /** Select the kernel path (QESMC_PATH_*); the default picks the fastest. */
+  void setPath(int path) { path_ = path; }
+
+  void solve(WINDSGeneralData *WGD, const int &itermax) override
+  {
+    qesmc_grid grid{domain.nx(), domain.ny(), domain.nz(), domain.dx(), domain.dy(), domain.dz_array.data()};
+    qesmc_coeffs coeffs{WGD->e.data(), WGD->f.data(), WGD->g.data(), WGD->h.data(),
+                        WGD->m.data(), WGD->n.data(), WGD->icellflag.data()};
+    qesmc_fields fields{WGD->u0.data(), WGD->v0.data(), WGD->w0.data(),
+                        WGD->u.data(),  WGD->v.data(),  WGD->w.data()};
+    qesmc_params params;
+    qesmc_default_params(&params);
+    params.omega = omega;
+    params.tol = tol;
+    params.itermax = itermax;
+    params.alpha1 = alpha1;
+    params.alpha2 = alpha2;
+    params.path = path_;
+    const int rc = qesmc_solve(ctx_, &grid, &coeffs, &fields, lambda.data(), &params, &last_);
+    if (rc != QESMC_OK) QESout::error(std::string("[Solver_HIP] ") + qesmc_status_string(rc));
+    std::printf("[Solver]\t Residual after %d itertations: %2.9f\n", last_.iterations,
+                last_.final_max_dlambda);
+    std::printf("\t\t elapsed time: %f s\n", last_.ms_total * 1e-3);
+  }
+
+private:
+  qesmc_ctx *ctx_ = nullptr;
+  qesmc_result last_{};
+  int path_ = QESMC_PATH_AUTO;
+};

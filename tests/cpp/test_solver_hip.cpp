@@ -1,0 +1,98 @@
+// Drives Solver_HIP the way qesWinds does (qesWinds/qesWindsMain.cpp:140-170):
+// a solver from SolverFactory by solveType, then per time step
+// resetLambda (or not) + solve(WGD, itermax), and checks u, v, w and lambda
+// bit for bit against the CPU oracle (test infrastructure, linked here only
+// as the checker).  Prints PASS and returns 0 on success.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "SolverFactory.h"
+#include "Solver_HIP.h"
+#include "WINDSGeneralData.h"
+#include "qesmc.h"
+
+extern "C" int oracle_solve_rb_f32(int nx, int ny, int nz, float dx, float dy, const float *dz_array,
+                                   const float *e, const float *f, const float *g, const float *h,
+                                   const float *m, const float *n, const int *icellflag, const float *u0,
+                                   const float *v0, const float *w0, float *R, float *lambda,
+                                   float *lambda_old, float omega, float tol, int itermax, int alpha1,
+                                   int alpha2, float *u, float *v, float *w, float *max_error_out);
+
+static int failures = 0;
+
+static void expect_equal(const std::vector<float> &a, const std::vector<float> &b, const char *what)
+{
+  size_t bad = 0;
+  for (size_t q = 0; q < a.size(); ++q)
+    if (!(a[q] == b[q]) && !(std::isnan(a[q]) && std::isnan(b[q]))) ++bad;
+  if (bad) {
+    std::printf("MISMATCH %s: %zu of %zu entries differ\n", what, bad, a.size());
+    ++failures;
+  }
+}
+
+static void set_inflow(WINDSGeneralData &wgd, const qes::Domain &dom, float wind_dir,
+                       const std::vector<qesmc_building> &bl)
+{
+  qesmc_inflow inflow{1, 3.8f, 10.0f, 0.5f, wind_dir};
+  const int rc = qesmc_city_build(dom.nx(), dom.ny(), dom.nz(), dom.dx(), dom.dz_array.data(), (int)bl.size(),
+                                  bl.data(), &inflow, wgd.e.data(), wgd.f.data(), wgd.g.data(), wgd.h.data(),
+                                  wgd.m.data(), wgd.n.data(), wgd.icellflag.data(), wgd.u0.data(),
+                                  wgd.v0.data(), wgd.w0.data());
+  if (rc != QESMC_OK) {
+    std::printf("city build failed: %s\n", qesmc_status_string(rc));
+    std::exit(2);
+  }
+}
+
+int main()
+{
+  // 90 x 70 x 24 cells, dx = dy = 2 m, stretched dz (1 m at ratio 1.03)
+  qes::Domain dom(90, 70, 24, 2.0f, 2.0f, 1.0f);
+  std::vector<float> dzc(24);
+  for (int k = 0; k < 24; ++k) dzc[k] = (float)std::pow(1.03, k);
+  dom.setVerticalStretching(dzc);
+  std::vector<qesmc_building> bl(12);
+  const int nb = qesmc_city_random_buildings(99, dom.nx(), dom.ny(), dom.dx(), 12, 6.0f, 24.0f, 4.0f, 20.0f,
+                                             0, 0.0f, 2, bl.data());
+  bl.resize(nb);
+
+  WINDSGeneralData wgd(dom);
+  std::unique_ptr<Solver> solver(SolverFactory::create(HIP_MI355X, dom, 1e-9f));
+  auto *hip = dynamic_cast<Solver_HIP *>(solver.get());
+
+  const long nc = dom.numCellCentered(), nf = dom.numFaceCentered();
+  std::vector<float> R(nc), lam(nc, 0.f), lam_old(nc), u(nf), v(nf), w(nf);
+  float me = 0.f;
+  const float dirs[3] = {270.0f, 247.5f, 300.0f};
+  const bool reset[3] = {true, false, true};
+  const int iters[3] = {300, 120, 0};
+  for (int step = 0; step < 3; ++step) {
+    set_inflow(wgd, dom, dirs[step], bl);
+    if (reset[step]) {
+      solver->resetLambda();
+      std::fill(lam.begin(), lam.end(), 0.f);
+    }
+    solver->solve(&wgd, iters[step]);
+    const int it = oracle_solve_rb_f32(dom.nx(), dom.ny(), dom.nz(), dom.dx(), dom.dy(), dom.dz_array.data(),
+                                       wgd.e.data(), wgd.f.data(), wgd.g.data(), wgd.h.data(), wgd.m.data(),
+                                       wgd.n.data(), wgd.icellflag.data(), wgd.u0.data(), wgd.v0.data(),
+                                       wgd.w0.data(), R.data(), lam.data(), lam_old.data(), 1.78f, 1e-9f,
+                                       iters[step], 1, 1, u.data(), v.data(), w.data(), &me);
+    if (hip->lastResult().iterations != it) {
+      std::printf("MISMATCH iterations step %d: %d vs %d\n", step, hip->lastResult().iterations, it);
+      ++failures;
+    }
+    expect_equal(wgd.u, u, "u");
+    expect_equal(wgd.v, v, "v");
+    expect_equal(wgd.w, w, "w");
+    expect_equal(solver->getLambda(), lam, "lambda");
+    std::printf("step %d: %d iterations, %.3f ms on device, %.1f GLUPS\n", step, hip->lastResult().iterations,
+                hip->lastResult().ms_total, hip->lastResult().glups);
+  }
+  std::printf(failures ? "FAIL\n" : "PASS\n");
+  return failures ? 1 : 0;
+}
