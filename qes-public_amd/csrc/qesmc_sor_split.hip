@@ -88,7 +88,8 @@ __device__ inline float wave_max(float v)
 template <bool MASKED, int VQ>
 __global__ __launch_bounds__(256) void k_sor_split(Dims d, int colour, int kb, int tiles_per_plane,
                                                    long long ntiles, float *__restrict__ own,
-                                                   float *oth, const float *__restrict__ Rc,
+                                                   const float *__restrict__ oth,
+                                                   const float *__restrict__ Rc,
                                                    const unsigned char *__restrict__ mask,
                                                    const unsigned *__restrict__ canon,
                                                    const float *__restrict__ ce,
@@ -126,6 +127,7 @@ __global__ __launch_bounds__(256) void k_sor_split(Dims d, int colour, int kb, i
     if (j > d.NY - 2) continue;
     const int k0 = 1 + kc * kb;
     const int k1 = k0 + kb < d.NZ - 1 ? k0 + kb : d.NZ - 1;
+#pragma unroll 1
     for (int k = k0; k < k1; ++k) {
       const int s = (j + k + colour) & 1;
       const long long x = (long long)(k * d.NY + j) * d.HP + q;
@@ -179,11 +181,21 @@ __global__ __launch_bounds__(256) void k_sor_split(Dims d, int colour, int kb, i
       // positions that are not updated are written back unchanged: every
       // half-cell of this row segment belongs to this lane in this pass
       Vec<VQ>::store(own + x, o);
-      if (k == 1) {
+      // k = 0 mirror (Solver_CPU_RB.cpp:154-162), kept in the other colour:
+      // own plane 0 at this half-index is ghost (i0,j,0), whose only reader is
+      // cell (i0,j,1) of the other colour, read in the *next* pass.  Its value
+      // must be lambda(i0,j,1) at the end of the previous iteration, which
+      // is oth plane 1 right now (that colour is not relaxed until the next
+      // pass).  In the first red pass the ghosts keep their initial values,
+      // as the reference mirrors only after an iteration.
+      if (k == 1 && !(colour == 0 && t == 0)) {
 #pragma unroll
-        for (int v = 0; v < VQ; ++v)
-          if (upd[v]) oth[x - d.splane + v] = nvv[v];  // k = 0 mirror of cell (i,j,1)
+        for (int v = 0; v < VQ; ++v) {
+          const int i0 = 2 * (q + v) + (s ^ 1);
+          if (i0 >= 1 && i0 <= d.NX - 2) own[x - d.splane + v] = c[v];
+        }
       }
+      (void)nvv;
     }
   }
   // block max -> one striped slot

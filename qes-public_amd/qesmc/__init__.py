@@ -15,7 +15,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libqesmc.so")
+# QESMC_LIB may point at another build of the same library (A/B timing runs)
+LIB_PATH = os.environ.get("QESMC_LIB") or os.path.join(_HERE, "libqesmc.so")
 
 OK = 0
 ERR_ARG = -1
