@@ -186,6 +186,67 @@ int qesmc_city_random_buildings(unsigned long long seed, int nx, int ny, float d
                                 float lmin, float lmax, float hmin, float hmax, int lognormal,
                                 float hcap, int margin, qesmc_building *out);
 
+/* ---- learned warm start: 3-D U-Net predicting lambda0 --------------------
+ *
+ * No counterpart in the reference (whose solvers start from lambda = 0 or the
+ * previous time step, Solver.h:115-120).  A U-Net maps the cell-type mask, R
+ * and the inflow profile of a resident problem to an initial lambda; SOR then
+ * runs to the same stopping rule, so accuracy is unchanged and only the
+ * iteration count changes.  BF16 activations/weights on MFMA with FP32
+ * accumulation; FP32 master weights and Adam.  Volumes are the ncx x ncy x ncz
+ * physical cells (k = 1..ncz), padded to multiples of 2^(levels-1). */
+typedef struct qesmc_unet qesmc_unet;
+
+typedef struct qesmc_unet_config {
+  int levels;          /* resolution levels (4 in the north-star network)    */
+  int base_channels;   /* channels at level 0, doubled per level (16)        */
+  int ncx, ncy, ncz;   /* problem size in cells (fixes the activation sizes) */
+  unsigned long long seed; /* weight initialisation                           */
+  float target_scale;  /* network output unit: lambda0 = output * target_scale *
+                          rms(R); 0 = set from the data at the first training
+                          call (RMS of lambda / rms(R) over the samples)     */
+} qesmc_unet_config;
+
+/* The output unit in use (see qesmc_unet_config.target_scale). */
+int qesmc_unet_target_scale(const qesmc_unet *net, float *scale);
+
+/* Number of input feature channels (mask, R, inflow u, inflow v, height, pad). */
+#define QESMC_UNET_IN_CHANNELS 8
+
+int qesmc_unet_create(qesmc_unet **out, int device, const qesmc_unet_config *cfg);
+int qesmc_unet_destroy(qesmc_unet *net);
+/* FP32 master parameters, layer by layer: for every 3x3x3 conv W[cout][27][cin]
+ * (tap = dz*9 + dy*3 + dx) then b[cout]; last the 1x1x1 head W[c0], b[1]. */
+int qesmc_unet_num_params(const qesmc_unet *net, long long *count);
+int qesmc_unet_get_params(qesmc_unet *net, float *host, long long count);
+int qesmc_unet_set_params(qesmc_unet *net, const float *host, long long count);
+/* Padded volume dims (x, y, z) of level 0. */
+int qesmc_unet_dims(const qesmc_unet *net, int *dx, int *dy, int *dz);
+
+/* Raw network access (tests): input is the padded volume, channels last
+ * [dz][dy][dx][QESMC_UNET_IN_CHANNELS] fp32 (stored as BF16 on the device);
+ * output [dz][dy][dx] fp32.  backward takes dL/doutput after a forward and
+ * returns dL/dparams in the parameter order above. */
+int qesmc_unet_forward(qesmc_unet *net, const float *input, float *output);
+int qesmc_unet_backward(qesmc_unet *net, const float *grad_output, float *grad_params);
+
+/* Training data from a resident solver context: features of its geometry and
+ * inflow, target = its current lambda (run it to convergence first). */
+int qesmc_unet_add_sample(qesmc_unet *net, qesmc_ctx *ctx);
+int qesmc_unet_num_samples(const qesmc_unet *net, int *count);
+int qesmc_unet_clear_samples(qesmc_unet *net);
+/* Adam steps on samples [first, last) drawn uniformly (batch of one sample),
+ * learning rate lr; mean loss over the steps.  ms: device time. */
+int qesmc_unet_train(qesmc_unet *net, int steps, float lr, int first, int last,
+                     unsigned long long seed, float *mean_loss, double *ms);
+/* Mean loss over samples [first, last) (no update). */
+int qesmc_unet_eval(qesmc_unet *net, int first, int last, float *mean_loss);
+/* Predict lambda0 for the resident problem of ctx and install it as the
+ * initial lambda of the next qesmc_run (boundary ring and top kept at 0, so
+ * the converged solution is unchanged).  ms: device time of the inference;
+ * tflops: achieved BF16 MFMA rate of its convolutions. */
+int qesmc_unet_predict(qesmc_unet *net, qesmc_ctx *ctx, double *ms, double *tflops);
+
 #ifdef __cplusplus
 }
 #endif

@@ -318,6 +318,131 @@ class Context:
         return u, v, w, lam_io, r.as_dict()
 
 
+UNET_IN_CHANNELS = 8
+
+
+class UNetConfig(C.Structure):
+    _fields_ = [("levels", C.c_int), ("base_channels", C.c_int), ("ncx", C.c_int), ("ncy", C.c_int),
+                ("ncz", C.c_int), ("seed", C.c_ulonglong), ("target_scale", C.c_float)]
+
+
+_UNET_SIGS = {
+    "qesmc_unet_create": [C.POINTER(C.c_void_p), C.c_int, C.POINTER(UNetConfig)],
+    "qesmc_unet_destroy": [C.c_void_p],
+    "qesmc_unet_num_params": [C.c_void_p, C.POINTER(C.c_longlong)],
+    "qesmc_unet_get_params": [C.c_void_p, _fp, C.c_longlong],
+    "qesmc_unet_set_params": [C.c_void_p, _fp, C.c_longlong],
+    "qesmc_unet_dims": [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "qesmc_unet_forward": [C.c_void_p, _fp, _fp],
+    "qesmc_unet_backward": [C.c_void_p, _fp, _fp],
+    "qesmc_unet_add_sample": [C.c_void_p, C.c_void_p],
+    "qesmc_unet_num_samples": [C.c_void_p, C.POINTER(C.c_int)],
+    "qesmc_unet_clear_samples": [C.c_void_p],
+    "qesmc_unet_train": [C.c_void_p, C.c_int, C.c_float, C.c_int, C.c_int, C.c_ulonglong,
+                         C.POINTER(C.c_float), C.POINTER(C.c_double)],
+    "qesmc_unet_eval": [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_float)],
+    "qesmc_unet_predict": [C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)],
+    "qesmc_unet_target_scale": [C.c_void_p, C.POINTER(C.c_float)],
+}
+EXPORTS = EXPORTS + tuple(_UNET_SIGS)
+
+
+def _unet_lib():
+    L = lib()
+    if not getattr(L, "_unet_sigs_set", False):
+        for name, args in _UNET_SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = C.c_int
+        L._unet_sigs_set = True
+    return L
+
+
+class UNet:
+    """3-D U-Net warm start (qesmc_unet_*): BF16 MFMA convolutions on the GPU."""
+
+    def __init__(self, ncx, ncy, ncz, levels=4, base_channels=16, seed=0, device=0, target_scale=0.0):
+        L = _unet_lib()
+        self._h = C.c_void_p()
+        cfg = UNetConfig(levels, base_channels, ncx, ncy, ncz, seed, target_scale)
+        _check(L.qesmc_unet_create(C.byref(self._h), int(device), C.byref(cfg)), "qesmc_unet_create")
+        n = C.c_longlong()
+        _check(L.qesmc_unet_num_params(self._h, C.byref(n)), "qesmc_unet_num_params")
+        self.nparams = n.value
+        dx, dy, dz = C.c_int(), C.c_int(), C.c_int()
+        _check(L.qesmc_unet_dims(self._h, C.byref(dx), C.byref(dy), C.byref(dz)), "qesmc_unet_dims")
+        self.dims = (dx.value, dy.value, dz.value)
+        self.levels, self.base_channels = levels, base_channels
+        self.ncx, self.ncy, self.ncz = int(ncx), int(ncy), int(ncz)
+
+    def close(self):
+        if self._h:
+            _unet_lib().qesmc_unet_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def target_scale(self):
+        s = C.c_float()
+        _check(_unet_lib().qesmc_unet_target_scale(self._h, C.byref(s)), "qesmc_unet_target_scale")
+        return s.value
+
+    def get_params(self):
+        out = np.empty(self.nparams, np.float32)
+        _check(_unet_lib().qesmc_unet_get_params(self._h, _f(out), self.nparams), "qesmc_unet_get_params")
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        _check(_unet_lib().qesmc_unet_set_params(self._h, _f(p), p.size), "qesmc_unet_set_params")
+
+    def forward(self, x):
+        """x: [dz, dy, dx, 8] fp32 -> [dz, dy, dx] fp32."""
+        dx, dy, dz = self.dims
+        x = np.ascontiguousarray(x, np.float32).reshape(dz, dy, dx, UNET_IN_CHANNELS)
+        y = np.empty((dz, dy, dx), np.float32)
+        _check(_unet_lib().qesmc_unet_forward(self._h, _f(x), _f(y)), "qesmc_unet_forward")
+        return y
+
+    def backward(self, grad_out):
+        g = np.ascontiguousarray(grad_out, np.float32)
+        gp = np.empty(self.nparams, np.float32)
+        _check(_unet_lib().qesmc_unet_backward(self._h, _f(g), _f(gp)), "qesmc_unet_backward")
+        return gp
+
+    def add_sample(self, ctx: "Context"):
+        _check(_unet_lib().qesmc_unet_add_sample(self._h, ctx._h), "qesmc_unet_add_sample")
+
+    def num_samples(self):
+        n = C.c_int()
+        _check(_unet_lib().qesmc_unet_num_samples(self._h, C.byref(n)), "qesmc_unet_num_samples")
+        return n.value
+
+    def clear_samples(self):
+        _check(_unet_lib().qesmc_unet_clear_samples(self._h), "qesmc_unet_clear_samples")
+
+    def train(self, steps, lr, first, last, seed=0):
+        loss, ms = C.c_float(), C.c_double()
+        _check(_unet_lib().qesmc_unet_train(self._h, int(steps), float(lr), int(first), int(last), int(seed),
+                                            C.byref(loss), C.byref(ms)), "qesmc_unet_train")
+        return loss.value, ms.value
+
+    def eval(self, first, last):
+        loss = C.c_float()
+        _check(_unet_lib().qesmc_unet_eval(self._h, int(first), int(last), C.byref(loss)), "qesmc_unet_eval")
+        return loss.value
+
+    def predict(self, ctx: "Context"):
+        """Install lambda0 for ctx's resident problem; returns (ms, TFLOP/s)."""
+        ms, tf = C.c_double(), C.c_double()
+        _check(_unet_lib().qesmc_unet_predict(self._h, ctx._h, C.byref(ms), C.byref(tf)), "qesmc_unet_predict")
+        return ms.value, tf.value
+
+
 def gpu_available() -> bool:
     """True when a HIP device can be opened through the library."""
     try:

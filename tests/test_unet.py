@@ -1,0 +1,175 @@
+"""The warm-start U-Net: the FP64 oracle pinned by finite differences (CPU),
+the device network's forward and backward against it (GPU), training, and
+that a warm start leaves the converged solution unchanged."""
+import numpy as np
+import pytest
+import torch
+
+
+def rand_input(shape_zyx, seed):
+    rng = np.random.default_rng(seed)
+    dz, dy, dx = shape_zyx
+    x = np.zeros((dz, dy, dx, 8), np.float32)
+    x[..., 0] = rng.random((dz, dy, dx)) > 0.3
+    x[..., 1:5] = rng.uniform(-1, 1, (dz, dy, dx, 4))
+    return x
+
+
+def he_params(levels, c0, seed):
+    import unet_oracle as U
+
+    layers, head, n = U.layout(levels, c0)
+    rng = np.random.default_rng(seed)
+    p = np.zeros(n)
+    for name, cin, cout, woff, boff in layers:
+        p[woff:woff + 27 * cin * cout] = rng.normal(0, np.sqrt(2.0 / (27 * cin)), 27 * cin * cout)
+        p[boff:boff + cout] = rng.normal(0, 0.05, cout)
+    p[head:head + c0 + 1] = rng.normal(0, 0.25, c0 + 1)
+    return p
+
+
+def test_oracle_layout_counts():
+    import unet_oracle as U
+
+    layers, head, n = U.layout(4, 16)
+    # the north-star network: 4 levels, 16/32/64/128 channels
+    assert [(l[1], l[2]) for l in layers] == [(8, 16), (16, 16), (16, 32), (32, 32), (32, 64), (64, 64), (64, 128),
+                                              (128, 128), (192, 64), (64, 64), (96, 32), (32, 32), (48, 16),
+                                              (16, 16)]
+    assert n == head + 17 and 1_400_000 < n < 1_500_000
+
+
+def test_oracle_gradient_matches_finite_differences():
+    """Pins the oracle's backward (torch autograd, float64) with central
+    differences on a 2-level network."""
+    import unet_oracle as U
+
+    levels, c0 = 2, 16
+    p = he_params(levels, c0, 1)
+    x = rand_input((4, 6, 8), 2)
+    g = np.random.default_rng(3).standard_normal((4, 6, 8))
+    grad = U.param_grad_np(p, x, g, levels, c0)
+    layers, head, n = U.layout(levels, c0)
+    rng = np.random.default_rng(4)
+    idx = list(rng.choice(n, 12, replace=False)) + [layers[0][3], layers[-1][4], head, head + c0]
+    h = 1e-6
+    for q in idx:
+        pp, pm = p.copy(), p.copy()
+        pp[q] += h
+        pm[q] -= h
+        fd = ((U.forward_np(pp, x, levels, c0) * g).sum() - (U.forward_np(pm, x, levels, c0) * g).sum()) / (2 * h)
+        assert abs(fd - grad[q]) <= 1e-6 * max(1.0, abs(fd)), (q, fd, grad[q])
+
+
+def test_oracle_equivariances():
+    """Pooling/upsampling bookkeeping: a constant input through a network
+    whose convolutions are identity-centred gives a constant output."""
+    import unet_oracle as U
+
+    levels, c0 = 3, 16
+    layers, head, n = U.layout(levels, c0)
+    p = np.zeros(n)
+    for name, cin, cout, woff, boff in layers:
+        W = np.zeros((cout, 27, cin))
+        for o in range(cout):
+            W[o, 13, o % cin] = 1.0  # centre tap copies a channel
+        p[woff:woff + 27 * cin * cout] = W.ravel()
+    p[head] = 1.0
+    x = np.zeros((8, 8, 8, 8))
+    x[..., 0] = 0.7
+    y = U.forward_np(p, x, levels, c0)
+    assert np.allclose(y, 0.7)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("levels,c0,dims", [(2, 16, (16, 12, 8)), (3, 16, (16, 16, 8)), (4, 16, (32, 16, 16)),
+                                            (3, 32, (16, 8, 16)), (4, 16, (32, 32, 32))])
+def test_device_forward_backward_match_oracle(qesmc, levels, c0, dims):
+    import unet_oracle as U
+
+    dx, dy, dz = dims
+    net = qesmc.UNet(dx, dy, dz, levels=levels, base_channels=c0, seed=7)
+    assert net.dims == (dx, dy, dz)
+    layers, head, n = U.layout(levels, c0)
+    assert net.nparams == n
+    p = he_params(levels, c0, 11).astype(np.float32)
+    net.set_params(p)
+    assert np.array_equal(net.get_params(), p)
+    x = rand_input((dz, dy, dx), 12)
+    y = net.forward(x)
+    ref = U.forward_np(p, x, levels, c0, bf16=True)
+    ref64 = U.forward_np(p, x, levels, c0, bf16=False)
+    scale = np.abs(ref64).max()
+    assert scale > 0.1
+    assert np.abs(y - ref).max() <= 2e-2 * scale, np.abs(y - ref).max() / scale
+    assert np.abs(y - ref64).max() <= 6e-2 * scale
+    g = np.random.default_rng(13).standard_normal((dz, dy, dx)).astype(np.float32)
+    gp = net.backward(g)
+    gref = U.param_grad_np(p, x, g, levels, c0, bf16=True)
+    gref64 = U.param_grad_np(p, x, g, levels, c0, bf16=False)
+    report, bad = [], []
+    for name, cin, cout, woff, boff in layers:
+        for part, a, b, b64 in (("W", gp[woff:woff + 27 * cin * cout], gref[woff:woff + 27 * cin * cout],
+                                 gref64[woff:woff + 27 * cin * cout]),
+                                ("b", gp[boff:boff + cout], gref[boff:boff + cout], gref64[boff:boff + cout])):
+            cos = float(np.dot(a, b) / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-30))
+            rel = float(np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30))
+            rel64 = float(np.linalg.norm(a - b64) / (np.linalg.norm(b64) + 1e-30))
+            report.append(f"{name}.{part}: cos {cos:.5f} rel {rel:.4f} rel_fp64 {rel64:.4f}")
+            # BF16 activations and activation gradients: rounding flips in the
+            # small deep-level sums leave a few per cent of noise; an indexing
+            # or layout error shows as a cosine far below 0.9
+            if not (cos >= 0.99 and rel <= 0.15):
+                bad.append(report[-1])
+    print("\n".join(report))
+    assert not bad, bad
+    a, b = gp[head:], gref[head:]
+    assert np.linalg.norm(a - b) <= 0.05 * np.linalg.norm(b)
+    net.close()
+
+
+def small_city(O, Q, seed, ncx=32, ncy=32, ncz=16, wind=None):
+    rng = np.random.default_rng(seed)
+    blds = Q.random_buildings(seed, ncx, ncy, 2.0, 4, 6.0, 16.0, 4.0, 14.0, margin=2)
+    wind = float(rng.uniform(0, 360)) if wind is None else wind
+    return O.build_city(ncx, ncy, 2.0, Q.dz_geometric(ncz, 1.0, 1.02), blds, (1, 3.8, 10.0, 0.5, wind))
+
+
+@pytest.mark.gpu
+def test_training_reduces_loss_and_warm_start_keeps_solution(qesmc, oracle):
+    ncx, ncy, ncz = 32, 32, 16
+    net = qesmc.UNet(ncx, ncy, ncz, levels=3, base_channels=16, seed=5)
+    with qesmc.Context(0) as ctx:
+        for s in range(12):
+            p = small_city(oracle, qesmc, 100 + s)
+            ctx.set_geometry(p)
+            ctx.set_inflow(p.u0, p.v0, p.w0)
+            ctx.set_lambda(None)
+            ctx.run(itermax=800)
+            net.add_sample(ctx)
+        assert net.num_samples() == 12
+        net.train(0, 2e-3, 0, 12)  # zero steps: fixes the output unit from the data
+        assert net.target_scale() > 0
+        l0 = net.eval(0, 12)
+        loss, ms = net.train(300, 2e-3, 0, 12, seed=1)
+        l1 = net.eval(0, 12)
+        assert l1 < 0.5 * l0, (l0, l1)
+        # warm start from the prediction: ring and top stay 0, the converged
+        # field is the cold-start one up to the fp32 floor
+        p = small_city(oracle, qesmc, 999)
+        ctx.set_geometry(p)
+        ctx.set_inflow(p.u0, p.v0, p.w0)
+        ms, tflops = net.predict(ctx)
+        lam0 = ctx.get_lambda()
+        NX, NY, NZ = p.nx - 1, p.ny - 1, p.nz - 1
+        L0 = lam0.reshape(NZ, NY, NX)
+        assert np.all(L0[:, :, 0] == 0) and np.all(L0[:, :, -1] == 0)
+        assert np.all(L0[:, 0, :] == 0) and np.all(L0[:, -1, :] == 0) and np.all(L0[-1] == 0)
+        assert np.array_equal(L0[0], L0[1])
+        assert np.abs(L0).max() > 0
+        ctx.run(itermax=3000)
+        uw, vw, ww = ctx.get_velocity()
+    u64, v64, w64, *_ = oracle.solve_rb_f64(p, tol=1e-10)
+    err = max(np.abs(uw - u64).max(), np.abs(vw - v64).max(), np.abs(ww - w64).max())
+    assert err <= 1e-4, err
+    net.close()
