@@ -154,6 +154,25 @@ int qesmc_get_lambda(qesmc_ctx *ctx, float *lambda);
  * physical = 1 the dimensionally consistent one (vertical term / dz_k). */
 int qesmc_divergence_linf(qesmc_ctx *ctx, int physical, double *out);
 
+/* ---- batches and tuning --------------------------------------------------- */
+
+/* Solve n resident problems (one context each, same device, any sizes)
+ * together: every SOR colour pass of all of them is one kernel launch, each
+ * problem keeping its own stop test (the reference solves one domain per
+ * Solver::solve; this has no counterpart there).  With use_graph != 0 the
+ * iterations are replayed as a captured HIP graph of check_every iterations.
+ * Per problem the results equal qesmc_run's; results[i] (optional) receives
+ * them (their timings span the whole batch).  ms_loop (optional): device
+ * time of the batched iterations.  Split path only, tile_v 1. */
+int qesmc_run_batch(qesmc_ctx **ctxs, int n, const qesmc_params *params, qesmc_result *results,
+                    int use_graph, double *ms_loop);
+
+/* Time `iterations` SOR iterations of the resident problem for a set of
+ * candidate tilings (tile_k, tile_v, wg_per_cu) and keep the fastest as the
+ * context's default for tile fields left at 0.  lambda is restored.  best
+ * (optional) receives base with the chosen tile fields. */
+int qesmc_autotune(qesmc_ctx *ctx, const qesmc_params *base, int iterations, qesmc_params *best);
+
 /* ---- input builder (flat stair-step cities) ------------------------------ */
 
 /* One RectangularBuilding (RectangularBuilding.h:69-110). */

@@ -296,25 +296,42 @@ int qesmc_set_lambda(qesmc_ctx *x, const float *lambda)
   return QESMC_OK;
 }
 
-int qesmc_run(qesmc_ctx *x, const qesmc_params *pp, qesmc_result *res)
+}  // extern "C"
+
+namespace qesmc {
+
+float row_max(const unsigned *row)
+{
+  float mx = 0.f;
+  for (int q = 0; q < kErrSlots; ++q) {
+    float v;
+    std::memcpy(&v, row + q, 4);
+    mx = v > mx ? v : mx;
+  }
+  return mx;
+}
+
+// Validation, optional re-encoding, tiling, convergence rows, lambda packing
+// and R, on the context's current stream (ev[0] .. ev[1]).
+int run_prepare(qesmc_ctx *x, const qesmc_params *pp, RunState &st)
 {
   if (!x || !pp) return QESMC_ERR_ARG;
   if (!x->have_geom || !x->have_inflow) return QESMC_ERR_STATE;
-  qesmc_params p = *pp;
+  qesmc_params &p = st.p;
+  p = *pp;
   if (p.itermax < 0) return QESMC_ERR_ARG;
   if (p.check_every < 1) p.check_every = 64;
   if (p.path != QESMC_PATH_AUTO && p.path != QESMC_PATH_PLAIN && p.path != QESMC_PATH_SPLIT)
     return QESMC_ERR_ARG;
-  const int path = (p.path == QESMC_PATH_PLAIN) ? QESMC_PATH_PLAIN : QESMC_PATH_SPLIT;
+  if (p.tile_k < 0 || p.wg_per_cu < 0) return QESMC_ERR_ARG;
+  if (p.tile_v != 0 && p.tile_v != 1 && p.tile_v != 2 && p.tile_v != 4) return QESMC_ERR_ARG;
+  st.path = (p.path == QESMC_PATH_PLAIN) ? QESMC_PATH_PLAIN : QESMC_PATH_SPLIT;
   HIPCHK(hipSetDevice(x->device));
-
   if (p.flags & QESMC_RUN_REENCODE) {
     int rc = encode_stencil(x);
     if (rc != QESMC_OK) return rc;
   }
-  const bool masked = x->compressible;
-  if (p.tile_k < 0 || p.wg_per_cu < 0) return QESMC_ERR_ARG;
-  if (p.tile_v != 0 && p.tile_v != 1 && p.tile_v != 2 && p.tile_v != 4) return QESMC_ERR_ARG;
+  st.masked = x->compressible;
   x->kplanes = p.tile_k > 0 ? p.tile_k : x->kplanes_auto;
   x->tile_v = p.tile_v > 0 ? p.tile_v : x->tile_v_auto;
   x->wg_per_cu = p.wg_per_cu > 0 ? p.wg_per_cu : x->wg_per_cu_auto;
@@ -324,41 +341,83 @@ int qesmc_run(qesmc_ctx *x, const qesmc_params *pp, qesmc_result *res)
     dfree(x->d_err);
     x->err_cap = 0;
     const long long cap = rows > 1024 ? rows : 1024;
-    HIPCHK(dalloc(&x->d_err, cap * qesmc::kErrSlots));
+    HIPCHK(dalloc(&x->d_err, cap * kErrSlots));
     x->err_cap = (int)cap;
   }
-
   hipStream_t s = x->stream;
   HIPCHK(hipEventRecord(x->ev[0], s));
   HIPCHK(hipMemsetAsync(x->d_status, 0, sizeof(unsigned), s));
-  HIPCHK(hipMemsetAsync(x->d_err, 0, sizeof(unsigned) * (size_t)(rows * qesmc::kErrSlots), s));
+  HIPCHK(hipMemsetAsync(x->d_err, 0, sizeof(unsigned) * (size_t)(rows * kErrSlots), s));
   HIPCHK(qesmc_launch_set_u32(x, x->d_err, 0x3f800000u));  // row 0: 1.0f
-  if (path == QESMC_PATH_SPLIT) HIPCHK(qesmc_launch_pack(x, true));
-  HIPCHK(qesmc_launch_divergence(x, path == QESMC_PATH_SPLIT, p.alpha1));
+  if (st.path == QESMC_PATH_SPLIT) HIPCHK(qesmc_launch_pack(x, true));
+  HIPCHK(qesmc_launch_divergence(x, st.path == QESMC_PATH_SPLIT, p.alpha1));
   HIPCHK(hipEventRecord(x->ev[1], s));
+  st.first_runs = p.itermax > 0 && (1.0f > p.tol);
+  return QESMC_OK;
+}
 
-  auto row_max = [](const unsigned *row) {
-    float mx = 0.f;
-    for (int q = 0; q < qesmc::kErrSlots; ++q) {
-      float v;
-      std::memcpy(&v, row + q, 4);
-      mx = v > mx ? v : mx;
-    }
-    return mx;
-  };
+// Unpack, mirror, correction, zeroing (ev[2] .. ev[3]), then the iteration
+// count from the convergence rows: iteration t ran iff max(row t) > tol.
+int run_finish(qesmc_ctx *x, const RunState &st, int launched, qesmc_result *res)
+{
+  const qesmc_params &p = st.p;
+  hipStream_t s = x->stream;
+  HIPCHK(hipEventRecord(x->ev[2], s));
+  if (st.path == QESMC_PATH_SPLIT) HIPCHK(qesmc_launch_pack(x, false));
+  HIPCHK(qesmc_launch_finish(x, st.first_runs, p.alpha1, p.alpha2));
+  HIPCHK(hipEventRecord(x->ev[3], s));
+  HIPCHK(hipStreamSynchronize(s));
 
-  const bool first_runs = p.itermax > 0 && (1.0f > p.tol);
+  std::vector<unsigned> herr(((size_t)launched + 1) * kErrSlots);
+  HIPCHK(hipMemcpy(herr.data(), x->d_err, sizeof(unsigned) * herr.size(), hipMemcpyDeviceToHost));
+  int count = 0;
+  while (count < launched && row_max(&herr[(size_t)count * kErrSlots]) > p.tol) ++count;
+  const float last = row_max(&herr[(size_t)count * kErrSlots]);
+  unsigned stw = 0;
+  HIPCHK(hipMemcpy(&stw, x->d_status, sizeof(unsigned), hipMemcpyDeviceToHost));
+  const int rc = (stw & ST_NONFINITE) ? QESMC_ERR_NONFINITE : QESMC_OK;
+  if (res) {
+    float ms_total = 0.f, ms_sor = 0.f;
+    (void)hipEventElapsedTime(&ms_total, x->ev[0], x->ev[3]);
+    (void)hipEventElapsedTime(&ms_sor, x->ev[1], x->ev[2]);
+    res->iterations = count;
+    res->path = st.path;
+    res->compressed = (st.path == QESMC_PATH_SPLIT && st.masked) ? 1 : 0;
+    res->status = rc;
+    res->final_max_dlambda = last;
+    res->ms_total = ms_total;
+    res->ms_sor = ms_sor;
+    const double lups = (double)(x->d.nx - 3) * (x->d.ny - 3) * (x->d.nz - 3) * count;
+    res->glups = ms_sor > 0.f ? lups / (ms_sor * 1e6) : 0.0;
+    res->tile_k = x->kplanes;
+    res->tile_v = x->tile_v;
+    res->wg_per_cu = x->wg_per_cu;
+    res->reserved = 0;
+  }
+  return rc;
+}
+
+}  // namespace qesmc
+
+extern "C" {
+
+int qesmc_run(qesmc_ctx *x, const qesmc_params *pp, qesmc_result *res)
+{
+  qesmc::RunState st;
+  int rc = qesmc::run_prepare(x, pp, st);
+  if (rc != QESMC_OK) return rc;
+  const qesmc_params &p = st.p;
+  hipStream_t s = x->stream;
   int launched = 0;
-  if (first_runs) {
+  if (st.first_runs) {
     // Poll convergence one chunk behind the device: after chunk n is queued,
     // wait for chunk n-1's row (already computed or about to be).  The gate
     // inside the kernels makes iterations after convergence no-ops, so the
     // result never depends on when the host notices.
     int chunk = 0;
     bool stop = false;
-    int rc = QESMC_OK;
     for (int t = 0; t < p.itermax && !stop; ++t) {
-      hipError_t e = qesmc_launch_iteration(x, path, masked, t, p.itermax, p.omega, p.tol);
+      hipError_t e = qesmc_launch_iteration(x, st.path, st.masked, t, p.itermax, p.omega, p.tol);
       if (e != hipSuccess) {
         rc = map_hip(e);
         break;
@@ -375,7 +434,7 @@ int qesmc_run(qesmc_ctx *x, const qesmc_params *pp, qesmc_result *res)
         if (chunk >= 1) {
           const int prev = slot ^ 1;
           if ((e = hipEventSynchronize(x->ev[4 + prev])) != hipSuccess) { rc = map_hip(e); break; }
-          if (!(row_max(x->h_poll + prev * qesmc::kErrSlots) > p.tol)) stop = true;
+          if (!(qesmc::row_max(x->h_poll + prev * qesmc::kErrSlots) > p.tol)) stop = true;
         }
         ++chunk;
       }
@@ -385,41 +444,7 @@ int qesmc_run(qesmc_ctx *x, const qesmc_params *pp, qesmc_result *res)
       return rc;
     }
   }
-  HIPCHK(hipEventRecord(x->ev[2], s));
-  if (path == QESMC_PATH_SPLIT) HIPCHK(qesmc_launch_pack(x, false));
-  HIPCHK(qesmc_launch_finish(x, first_runs, p.alpha1, p.alpha2));
-  HIPCHK(hipEventRecord(x->ev[3], s));
-  HIPCHK(hipStreamSynchronize(s));
-
-  // iteration count: iteration t ran iff max(row t) > tol (row 0 = 1.0f)
-  std::vector<unsigned> herr(((size_t)launched + 1) * qesmc::kErrSlots);
-  HIPCHK(hipMemcpy(herr.data(), x->d_err, sizeof(unsigned) * herr.size(), hipMemcpyDeviceToHost));
-  int count = 0;
-  while (count < launched && row_max(&herr[(size_t)count * qesmc::kErrSlots]) > p.tol) ++count;
-  const float last = row_max(&herr[(size_t)count * qesmc::kErrSlots]);
-  unsigned st = 0;
-  HIPCHK(hipMemcpy(&st, x->d_status, sizeof(unsigned), hipMemcpyDeviceToHost));
-  const int rc = (st & qesmc::ST_NONFINITE) ? QESMC_ERR_NONFINITE : QESMC_OK;
-
-  if (res) {
-    float ms_total = 0.f, ms_sor = 0.f;
-    (void)hipEventElapsedTime(&ms_total, x->ev[0], x->ev[3]);
-    (void)hipEventElapsedTime(&ms_sor, x->ev[1], x->ev[2]);
-    res->iterations = count;
-    res->path = path;
-    res->compressed = (path == QESMC_PATH_SPLIT && masked) ? 1 : 0;
-    res->status = rc;
-    res->final_max_dlambda = last;
-    res->ms_total = ms_total;
-    res->ms_sor = ms_sor;
-    const double lups = (double)(x->d.nx - 3) * (x->d.ny - 3) * (x->d.nz - 3) * count;
-    res->glups = ms_sor > 0.f ? lups / (ms_sor * 1e6) : 0.0;
-    res->tile_k = x->kplanes;
-    res->tile_v = x->tile_v;
-    res->wg_per_cu = x->wg_per_cu;
-    res->reserved = 0;
-  }
-  return rc;
+  return qesmc::run_finish(x, st, launched, res);
 }
 
 int qesmc_get_velocity(qesmc_ctx *x, float *u, float *v, float *w)

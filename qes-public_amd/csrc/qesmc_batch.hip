@@ -1,0 +1,388 @@
+// Batched solves: many independent problems (uneven sizes, one context each)
+// relaxed by one kernel launch per colour pass, and the tile autotuner.
+//
+// Every instance keeps its own colour-split arrays, stencil encoding and
+// convergence rows; a device table describes them.  Wave tiles are numbered
+// over all instances (instance-major, then plane chunk, then position); a
+// wave finds its instance by binary search of the table's tile offsets,
+// checks that instance's gate (its iteration counter and its previous max
+// |dlambda|), relaxes the tile exactly as the single-problem kernel does and
+// publishes its max into the instance's striped slots.  Converged instances
+// cost one gate read per tile, so a batch behaves as if each problem were
+// solved alone.  Iterations can be replayed as a captured HIP graph: the
+// iteration number then comes from a device counter advanced by the graph.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "qesmc_internal.h"
+
+hipError_t qesmc_launch_set_u32(qesmc_ctx *x, unsigned *p, unsigned v);
+
+namespace qesmc {
+
+namespace {
+
+constexpr int kXcdsB = 8;
+
+struct BatchItem {
+  Dims d;
+  float *lc[2];
+  const float *Rc[2];
+  const unsigned char *mask[2];
+  const unsigned *canon;
+  const float *cc[2][6];
+  unsigned *err;
+  unsigned *status;
+  int masked;
+  int kb;
+  int tiles_per_plane;
+  int pad;
+  long long tile_begin;  // first global wave tile of this instance
+};
+
+__device__ inline float wave_max_b(float v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// t = t_base[0] + t_off when t_base is given (graph replay), else t_off.
+__global__ __launch_bounds__(256) void k_sor_batch(const BatchItem *__restrict__ items, int n, long long total,
+                                                   int colour, const int *t_base, int t_off, int itermax,
+                                                   float omega, float one_m_omega, float tol)
+{
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int t = (t_base ? t_base[0] : 0) + t_off;
+  if (t >= itermax) return;
+  const int xcd = blockIdx.x % kXcdsB;
+  const int local = blockIdx.x / kXcdsB;
+  const int nblk = (gridDim.x - xcd + kXcdsB - 1) / kXcdsB;
+  const long long per = (total + kXcdsB - 1) / kXcdsB;
+  const long long tbeg = xcd * per;
+  const long long tend = tbeg + per < total ? tbeg + per : total;
+  int cur = -1;
+  bool open = false;
+  for (long long wt = tbeg + (long long)local * 4 + wave; wt < tend; wt += (long long)nblk * 4) {
+    // instance of this tile (tile offsets are increasing)
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (items[mid].tile_begin <= wt) lo = mid;
+      else hi = mid - 1;
+    }
+    const BatchItem &it = items[lo];
+    if (lo != cur) {
+      cur = lo;
+      open = wave_max_b(__uint_as_float(it.err[(long long)t * kErrSlots + lane])) > tol;
+    }
+    if (!open) continue;
+    const Dims &d = it.d;
+    const long long lt = wt - it.tile_begin;
+    const int kc = (int)(lt / it.tiles_per_plane);
+    const int pt = (int)(lt - (long long)kc * it.tiles_per_plane);
+    const int p = pt * 64 + lane;
+    const int jr = p / d.HP;
+    const int j = jr + 1;
+    const int q = p - jr * d.HP;
+    float dl = 0.f;
+    if (j <= d.NY - 2) {
+      float *__restrict__ own = it.lc[colour];
+      const float *__restrict__ oth = it.lc[colour ^ 1];
+      const float *__restrict__ Rc = it.Rc[colour];
+      const int k0 = 1 + kc * it.kb;
+      const int k1 = k0 + it.kb < d.NZ - 1 ? k0 + it.kb : d.NZ - 1;
+      for (int k = k0; k < k1; ++k) {
+        const int s = (j + k + colour) & 1;
+        const long long x = (long long)(k * d.NY + j) * d.HP + q;
+        const int i = 2 * q + s;
+        const bool upd = (i >= 1 && i <= d.NX - 2);
+        const float o = own[x];
+        const float c = oth[x];
+        const float xm = s ? c : oth[x - 1];
+        const float xp = s ? oth[x + 1] : c;
+        const float yp = oth[x + d.HP], ym = oth[x - d.HP];
+        const float zp = oth[x + d.splane], zm = oth[x - d.splane];
+        float ec, fc, gc, hc, mc, nc;
+        if (it.masked) {
+          const unsigned m = it.mask[colour][x];
+          ec = (m & 1u) ? __uint_as_float(it.canon[0 * d.NZ + k]) : 0.f;
+          fc = (m & 2u) ? __uint_as_float(it.canon[1 * d.NZ + k]) : 0.f;
+          gc = (m & 4u) ? __uint_as_float(it.canon[2 * d.NZ + k]) : 0.f;
+          hc = (m & 8u) ? __uint_as_float(it.canon[3 * d.NZ + k]) : 0.f;
+          mc = (m & 16u) ? __uint_as_float(it.canon[4 * d.NZ + k]) : 0.f;
+          nc = (m & 32u) ? __uint_as_float(it.canon[5 * d.NZ + k]) : 0.f;
+        } else {
+          ec = upd ? it.cc[colour][0][x] : 0.f;
+          fc = upd ? it.cc[colour][1][x] : 0.f;
+          gc = upd ? it.cc[colour][2][x] : 0.f;
+          hc = upd ? it.cc[colour][3][x] : 0.f;
+          mc = upd ? it.cc[colour][4][x] : 0.f;
+          nc = upd ? it.cc[colour][5][x] : 0.f;
+        }
+        if (upd) {
+          const float nv = (omega / (ec + fc + gc + hc + mc + nc))
+                               * (ec * xp + fc * xm + gc * yp + hc * ym + mc * zp + nc * zm - Rc[x])
+                           + one_m_omega * o;
+          own[x] = nv;
+          if (!isfinite(nv)) atomicOr(it.status, ST_NONFINITE);
+          const float dd = fabsf(nv - o);
+          dl = (dd > dl) ? dd : dl;
+        }
+        // k = 0 mirror kept in the other colour (see the single-problem kernel)
+        if (k == 1 && !(colour == 0 && t == 0)) {
+          const int i0 = 2 * q + (s ^ 1);
+          if (i0 >= 1 && i0 <= d.NX - 2) own[x - d.splane] = c;
+        }
+      }
+    }
+    const float wm = wave_max_b(dl);
+    if (lane == 0 && wm > 0.f)
+      atomicMax(it.err + (long long)(t + 1) * kErrSlots + (wt % kErrSlots), __float_as_uint(wm));
+  }
+}
+
+__global__ void k_advance(int *t_base, int by) { t_base[0] += by; }
+
+// number of instances whose row t is still above tol (t = t_base[0] + t_off)
+__global__ __launch_bounds__(64) void k_count_active(const BatchItem *items, int n, const int *t_base, int t_off,
+                                                     int itermax, float tol, int *out)
+{
+  const int t = (t_base ? t_base[0] : 0) + t_off;
+  int c = 0;
+  if (t < itermax)
+    for (int i = threadIdx.x; i < n; i += 64) {
+      float mx = 0.f;
+      for (int q = 0; q < kErrSlots; ++q) mx = fmaxf(mx, __uint_as_float(items[i].err[(long long)t * kErrSlots + q]));
+      c += mx > tol ? 1 : 0;
+    }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+  if (threadIdx.x == 0) out[0] = c;
+}
+
+int map_err_b(hipError_t e)
+{
+  if (e == hipSuccess) return QESMC_OK;
+  if (e == hipErrorOutOfMemory) return QESMC_ERR_OOM;
+  return QESMC_ERR_HIP;
+}
+
+#define BCHK(expr)                                 \
+  do {                                             \
+    hipError_t _e = (expr);                        \
+    if (_e != hipSuccess) return map_err_b(_e);    \
+  } while (0)
+
+}  // namespace
+
+}  // namespace qesmc
+
+using namespace qesmc;
+
+extern "C" {
+
+int qesmc_run_batch(qesmc_ctx **ctxs, int n, const qesmc_params *params, qesmc_result *results, int use_graph,
+                    double *ms_loop)
+{
+  if (!ctxs || n < 1 || !params) return QESMC_ERR_ARG;
+  for (int i = 0; i < n; ++i) {
+    if (!ctxs[i]) return QESMC_ERR_ARG;
+    if (ctxs[i]->device != ctxs[0]->device) return QESMC_ERR_UNSUPPORTED;
+    for (int j = 0; j < i; ++j)
+      if (ctxs[j] == ctxs[i]) return QESMC_ERR_ARG;
+  }
+  if (params->path == QESMC_PATH_PLAIN) return QESMC_ERR_UNSUPPORTED;
+  if (params->tile_v > 1) return QESMC_ERR_UNSUPPORTED;  // the batched kernel relaxes one half-cell per lane
+  qesmc_ctx *lead = ctxs[0];
+  BCHK(hipSetDevice(lead->device));
+  hipStream_t bs = lead->stream;
+  std::vector<hipStream_t> own_streams(n);
+  for (int i = 0; i < n; ++i) {
+    own_streams[i] = ctxs[i]->stream;
+    BCHK(hipStreamSynchronize(own_streams[i]));
+    ctxs[i]->stream = bs;  // every phase of every instance on the batch stream
+  }
+  auto restore = [&]() {
+    for (int i = 0; i < n; ++i) ctxs[i]->stream = own_streams[i];
+  };
+  std::vector<RunState> st(n);
+  int rc = QESMC_OK;
+  for (int i = 0; i < n && rc == QESMC_OK; ++i) rc = run_prepare(ctxs[i], params, st[i]);
+  if (rc != QESMC_OK) {
+    (void)hipStreamSynchronize(bs);
+    restore();
+    return rc;
+  }
+  const qesmc_params &p = st[0].p;
+  // device table
+  std::vector<BatchItem> items(n);
+  long long total = 0;
+  for (int i = 0; i < n; ++i) {
+    qesmc_ctx *x = ctxs[i];
+    BatchItem &it = items[i];
+    std::memset(&it, 0, sizeof(it));
+    it.d = x->d;
+    for (int c = 0; c < 2; ++c) {
+      it.lc[c] = x->d_lc[c];
+      it.Rc[c] = x->d_Rc[c];
+      it.mask[c] = x->d_mask[c];
+      for (int a = 0; a < 6; ++a) it.cc[c][a] = x->d_cc[c][a];
+    }
+    it.canon = x->d_canon;
+    it.err = x->d_err;
+    it.status = x->d_status;
+    it.masked = st[i].masked ? 1 : 0;
+    it.kb = x->kplanes < 1 ? 1 : x->kplanes;
+    const long long flat = (long long)(x->d.NY - 2) * x->d.HP;
+    it.tiles_per_plane = (int)((flat + 63) / 64);
+    const int nkc = x->d.NZ - 2 >= 1 && x->d.NY - 2 >= 1 ? (x->d.NZ - 2 + it.kb - 1) / it.kb : 0;
+    it.tile_begin = total;
+    total += (long long)it.tiles_per_plane * nkc;
+  }
+  BatchItem *d_items = nullptr;
+  int *d_t = nullptr, *d_active = nullptr, *h_active = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto cleanup = [&]() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (d_items) (void)hipFree(d_items);
+    if (d_t) (void)hipFree(d_t);
+    if (d_active) (void)hipFree(d_active);
+    if (h_active) (void)hipHostFree(h_active);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  };
+  auto fail = [&](hipError_t e) {
+    (void)hipStreamSynchronize(bs);
+    cleanup();
+    restore();
+    return map_err_b(e);
+  };
+  hipError_t e;
+  if ((e = hipMalloc((void **)&d_items, sizeof(BatchItem) * n)) != hipSuccess) return fail(e);
+  if ((e = hipMalloc((void **)&d_t, sizeof(int))) != hipSuccess) return fail(e);
+  if ((e = hipMalloc((void **)&d_active, sizeof(int))) != hipSuccess) return fail(e);
+  if ((e = hipHostMalloc((void **)&h_active, sizeof(int), hipHostMallocDefault)) != hipSuccess) return fail(e);
+  if ((e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess) return fail(e);
+  if ((e = hipMemcpyAsync(d_items, items.data(), sizeof(BatchItem) * n, hipMemcpyHostToDevice, bs)) != hipSuccess)
+    return fail(e);
+  if ((e = qesmc_launch_set_u32(lead, (unsigned *)d_t, 0u)) != hipSuccess) return fail(e);
+
+  const int wg = p.wg_per_cu > 0 ? p.wg_per_cu : lead->wg_per_cu_auto;
+  long long blocks = (total + 3) / 4;
+  const long long cap = (long long)lead->n_cu * (wg < 1 ? 1 : wg);
+  if (blocks > cap) blocks = cap;
+  if (blocks < kXcdsB) blocks = kXcdsB;
+  const float omo = 1.0f - p.omega;
+  const bool any_runs = st[0].first_runs && total > 0;
+  const int C = p.check_every;
+  int launched = 0;
+  if ((e = hipEventRecord(e0, bs)) != hipSuccess) return fail(e);
+  if (any_runs) {
+    auto launch_iter = [&](const int *tb, int toff) {
+      for (int colour = 0; colour < 2; ++colour)
+        hipLaunchKernelGGL(k_sor_batch, dim3((unsigned)blocks), dim3(256), 0, bs, d_items, n, total, colour, tb, toff,
+                           p.itermax, p.omega, omo, p.tol);
+    };
+    if (use_graph) {
+      // one captured chunk of C iterations, advancing the device counter
+      if ((e = hipStreamBeginCapture(bs, hipStreamCaptureModeThreadLocal)) != hipSuccess) return fail(e);
+      for (int j = 0; j < C; ++j) launch_iter(d_t, j);
+      hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, bs, d_t, C);
+      if ((e = hipStreamEndCapture(bs, &graph)) != hipSuccess) return fail(e);
+      if ((e = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0)) != hipSuccess) return fail(e);
+    }
+    bool stop = false;
+    while (launched < p.itermax && !stop) {
+      const int chunk = std::min(C, p.itermax - launched);
+      if (use_graph) {
+        if ((e = hipGraphLaunch(gexec, bs)) != hipSuccess) return fail(e);
+      } else {
+        for (int j = 0; j < chunk; ++j) launch_iter(nullptr, launched + j);
+        if ((e = hipGetLastError()) != hipSuccess) return fail(e);
+      }
+      launched += use_graph ? C : chunk;
+      if (launched >= p.itermax) break;
+      // instances still open at the next iteration
+      hipLaunchKernelGGL(k_count_active, dim3(1), dim3(64), 0, bs, d_items, n, nullptr, launched, p.itermax, p.tol,
+                         d_active);
+      if ((e = hipMemcpyAsync(h_active, d_active, sizeof(int), hipMemcpyDeviceToHost, bs)) != hipSuccess)
+        return fail(e);
+      if ((e = hipStreamSynchronize(bs)) != hipSuccess) return fail(e);
+      if (*h_active == 0) stop = true;
+    }
+    if (launched > p.itermax) launched = p.itermax;
+  }
+  if ((e = hipEventRecord(e1, bs)) != hipSuccess) return fail(e);
+  for (int i = 0; i < n; ++i) {
+    const int r = run_finish(ctxs[i], st[i], any_runs ? launched : 0, results ? &results[i] : nullptr);
+    if (r != QESMC_OK && rc == QESMC_OK) rc = r;
+  }
+  if (ms_loop) {
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    *ms_loop = t;
+  }
+  cleanup();
+  restore();
+  return rc;
+}
+
+// Time `iterations` SOR iterations for each candidate tiling on the resident
+// problem and keep the fastest as the context's default (tile fields 0).
+// lambda is saved and restored, so the problem's state is unchanged.
+int qesmc_autotune(qesmc_ctx *x, const qesmc_params *base, int iterations, qesmc_params *best)
+{
+  if (!x || !base || iterations < 1) return QESMC_ERR_ARG;
+  if (!x->have_geom || !x->have_inflow) return QESMC_ERR_STATE;
+  BCHK(hipSetDevice(x->device));
+  float *saved = nullptr;
+  BCHK(hipMalloc((void **)&saved, sizeof(float) * (size_t)x->ncell));
+  BCHK(hipMemcpyAsync(saved, x->d_lam, sizeof(float) * (size_t)x->ncell, hipMemcpyDeviceToDevice, x->stream));
+  static const int cand[][3] = {{8, 1, 8}, {4, 1, 8}, {16, 1, 8}, {8, 1, 4}, {8, 1, 16},
+                                {4, 4, 8}, {8, 4, 8}, {8, 2, 8}, {2, 1, 8}, {32, 1, 8}};
+  double best_ms = 1e300;
+  int bi = 0, rc = QESMC_OK;
+  for (int c = 0; c < (int)(sizeof(cand) / sizeof(cand[0])); ++c) {
+    qesmc_params p = *base;
+    p.path = QESMC_PATH_SPLIT;
+    p.itermax = iterations;
+    p.tol = -1.0f;  // never converges: every candidate runs the same iterations
+    p.tile_k = cand[c][0];
+    p.tile_v = cand[c][1];
+    p.wg_per_cu = cand[c][2];
+    p.flags &= ~QESMC_RUN_REENCODE;
+    qesmc_result r;
+    double ms = 0.0;
+    for (int rep = 0; rep < 2 && rc == QESMC_OK; ++rep) {  // first repetition warms caches
+      BCHK(hipMemcpyAsync(x->d_lam, saved, sizeof(float) * (size_t)x->ncell, hipMemcpyDeviceToDevice, x->stream));
+      rc = qesmc_run(x, &p, &r);
+      ms = r.ms_sor;
+    }
+    if (rc != QESMC_OK) break;
+    if (ms < best_ms) {
+      best_ms = ms;
+      bi = c;
+    }
+  }
+  (void)hipMemcpyAsync(x->d_lam, saved, sizeof(float) * (size_t)x->ncell, hipMemcpyDeviceToDevice, x->stream);
+  (void)hipStreamSynchronize(x->stream);
+  (void)hipFree(saved);
+  if (rc != QESMC_OK) return rc;
+  x->kplanes_auto = cand[bi][0];
+  x->tile_v_auto = cand[bi][1];
+  x->wg_per_cu_auto = cand[bi][2];
+  if (best) {
+    *best = *base;
+    best->tile_k = cand[bi][0];
+    best->tile_v = cand[bi][1];
+    best->wg_per_cu = cand[bi][2];
+  }
+  return QESMC_OK;
+}
+
+}  // extern "C"

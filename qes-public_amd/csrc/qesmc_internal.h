@@ -81,3 +81,18 @@ struct qesmc_ctx {
   int wg_per_cu_auto = 8;
   int n_cu = 256;              // compute units of the device
 };
+
+namespace qesmc {
+
+// Phases of one solve shared by qesmc_run and qesmc_run_batch.
+struct RunState {
+  qesmc_params p{};
+  int path = QESMC_PATH_SPLIT;
+  bool masked = false;
+  bool first_runs = false;
+};
+int run_prepare(qesmc_ctx *x, const qesmc_params *pp, RunState &st);
+int run_finish(qesmc_ctx *x, const RunState &st, int launched, qesmc_result *res);
+float row_max(const unsigned *row);
+
+}  // namespace qesmc

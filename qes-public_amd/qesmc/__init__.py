@@ -318,6 +318,40 @@ class Context:
         return u, v, w, lam_io, r.as_dict()
 
 
+EXPORTS = EXPORTS + ("qesmc_run_batch", "qesmc_autotune")
+
+
+def _batch_lib():
+    L = lib()
+    if not getattr(L, "_batch_sigs_set", False):
+        L.qesmc_run_batch.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(Params), C.POINTER(Result), C.c_int,
+                                      C.POINTER(C.c_double)]
+        L.qesmc_run_batch.restype = C.c_int
+        L.qesmc_autotune.argtypes = [C.c_void_p, C.POINTER(Params), C.c_int, C.POINTER(Params)]
+        L.qesmc_autotune.restype = C.c_int
+        L._batch_sigs_set = True
+    return L
+
+
+def run_batch(ctxs, params: Params | None = None, use_graph=True, **kw):
+    """qesmc_run_batch over resident contexts: ([result dict], loop ms)."""
+    p = params if params is not None else default_params(**kw)
+    n = len(ctxs)
+    hs = (C.c_void_p * n)(*[c._h.value for c in ctxs])
+    res = (Result * n)()
+    ms = C.c_double()
+    rc = _batch_lib().qesmc_run_batch(hs, n, C.byref(p), res, 1 if use_graph else 0, C.byref(ms))
+    _check(rc, "qesmc_run_batch")
+    return [r.as_dict() for r in res], ms.value
+
+
+def autotune(ctx: "Context", iterations=40, params: Params | None = None, **kw) -> Params:
+    p = params if params is not None else default_params(**kw)
+    best = Params()
+    _check(_batch_lib().qesmc_autotune(ctx._h, C.byref(p), int(iterations), C.byref(best)), "qesmc_autotune")
+    return best
+
+
 UNET_IN_CHANNELS = 8
 
 
