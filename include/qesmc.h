@@ -205,6 +205,25 @@ int qesmc_city_random_buildings(unsigned long long seed, int nx, int ny, float d
                                 float lmin, float lmax, float hmin, float hmax, int lognormal,
                                 float hcap, int margin, qesmc_building *out);
 
+/* The same city built on the device, straight into the context: replaces the
+ * serial set-up of WINDSGeneralData's constructor for this case (flags
+ * WINDSGeneralData.cpp:439-444 and PolyBuilding.cpp:130-236, walls and
+ * coefficients Wall.cpp:44-349, 827-846, inflow WindProfilerSensorType.cpp
+ * :571-621, zeroing Wall.cpp:805-824) followed by qesmc_set_geometry and
+ * qesmc_set_inflow.  dy = dx.  The resident arrays are bitwise equal to
+ * qesmc_city_build's.  Building fields must be finite (|value| <= 1e8). */
+int qesmc_set_city(qesmc_ctx *ctx, int nx, int ny, int nz, float dx, const float *dz_array, int nbuildings,
+                   const qesmc_building *buildings, const qesmc_inflow *inflow);
+
+/* Replace u0/v0/w0 of the resident geometry by one sensor profile extruded
+ * over the domain with solid zeroing (a new wind direction or speed on the
+ * same city: WindProfilerSensorType.cpp:571-621, Wall.cpp:805-824). */
+int qesmc_set_city_inflow(qesmc_ctx *ctx, const qesmc_inflow *inflow);
+
+/* Copy the resident inputs back to the host; NULL pointers are skipped. */
+int qesmc_get_inputs(qesmc_ctx *ctx, float *e, float *f, float *g, float *h, float *m, float *n,
+                     int *icellflag, float *u0, float *v0, float *w0);
+
 /* ---- learned warm start: 3-D U-Net predicting lambda0 --------------------
  *
  * No counterpart in the reference (whose solvers start from lambda = 0 or the

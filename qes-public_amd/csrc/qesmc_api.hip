@@ -153,6 +153,33 @@ int encode_stencil(qesmc_ctx *x)
 
 }  // namespace
 
+namespace qesmc {
+
+int geometry_reserve(qesmc_ctx *x, int nx, int ny, int nz)
+{
+  if (x->d.nx != nx || x->d.ny != ny || x->d.nz != nz || !x->d_lam) {
+    HIPCHK(hipStreamSynchronize(x->stream));
+    free_geometry(x);
+    int rc = alloc_geometry(x, nx, ny, nz);
+    if (rc != QESMC_OK) {
+      free_geometry(x);
+      return rc;
+    }
+  }
+  return QESMC_OK;
+}
+
+int geometry_commit(qesmc_ctx *x)
+{
+  x->have_geom = false;
+  int rc = encode_stencil(x);
+  if (rc != QESMC_OK) return rc;
+  x->have_geom = true;
+  return QESMC_OK;
+}
+
+}  // namespace qesmc
+
 extern "C" {
 
 const char *qesmc_status_string(int s)
@@ -247,14 +274,9 @@ int qesmc_set_geometry(qesmc_ctx *x, const qesmc_grid *grid, const qesmc_coeffs 
   const long long ncell = (long long)(grid->nx - 1) * (grid->ny - 1) * (grid->nz - 1);
   if (ncell > (1LL << 31) - 1) return QESMC_ERR_UNSUPPORTED;  // per-plane int math
   HIPCHK(hipSetDevice(x->device));
-  if (x->d.nx != grid->nx || x->d.ny != grid->ny || x->d.nz != grid->nz || !x->d_lam) {
-    HIPCHK(hipStreamSynchronize(x->stream));
-    free_geometry(x);
-    int rc = alloc_geometry(x, grid->nx, grid->ny, grid->nz);
-    if (rc != QESMC_OK) {
-      free_geometry(x);
-      return rc;
-    }
+  {
+    int rc = qesmc::geometry_reserve(x, grid->nx, grid->ny, grid->nz);
+    if (rc != QESMC_OK) return rc;
   }
   x->dx = grid->dx;
   x->dy = grid->dy;
@@ -264,11 +286,7 @@ int qesmc_set_geometry(qesmc_ctx *x, const qesmc_grid *grid, const qesmc_coeffs 
   for (int a = 0; a < 6; ++a) HIPCHK(hipMemcpyAsync(x->d_coef[a], src[a], cb, hipMemcpyHostToDevice, x->stream));
   HIPCHK(hipMemcpyAsync(x->d_flag, cf->icellflag, sizeof(int) * (size_t)x->ncell, hipMemcpyHostToDevice, x->stream));
   HIPCHK(hipMemcpyAsync(x->d_dz, grid->dz_array, sizeof(float) * (grid->nz - 1), hipMemcpyHostToDevice, x->stream));
-  x->have_geom = false;
-  int rc = encode_stencil(x);
-  if (rc != QESMC_OK) return rc;
-  x->have_geom = true;
-  return QESMC_OK;
+  return qesmc::geometry_commit(x);
 }
 
 int qesmc_set_inflow(qesmc_ctx *x, const float *u0, const float *v0, const float *w0)

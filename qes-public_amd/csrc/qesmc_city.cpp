@@ -14,20 +14,10 @@
 #include <cstring>
 #include <vector>
 
-#include "qesmc.h"
+#include "qesmc_city_internal.h"
 
-namespace {
+namespace qesmc {
 
-struct Grid {
-  int nx, ny, nz;
-  long long sy, sz, fy, fz;
-  long long cell(int i, int j, int k) const { return i + j * sy + k * sz; }
-  long long face(int i, int j, int k) const { return i + j * fy + k * fz; }
-};
-
-inline bool is_solid(int fl) { return fl == 0 || fl == 2; }
-
-// Domain::defineVerticalGrid (Domain.cpp:146-160)
 void vertical_grid(int nz, const float *dz, std::vector<float> &z_face, std::vector<float> &z)
 {
   z_face.assign(nz, 0.f);
@@ -39,63 +29,108 @@ void vertical_grid(int nz, const float *dz, std::vector<float> &z_face, std::vec
   for (int k = 1; k < nz - 1; ++k) z[k] = 0.5f * (z_face[k] + z_face[k + 1]);
 }
 
+Footprint building_footprint(int nz, float dx, const std::vector<float> &z_face, const std::vector<float> &z,
+                             const qesmc_building &b)
+{
+  Footprint F;
+  const float dy = dx;  // getDomainSize() returns (dx, dx, dz), Domain.h:104
+  const float height_eff = b.height + b.base_height;
+  const float rot = (float)(b.rotation_deg * M_PI / 180.0);
+  F.xv[0] = F.xv[4] = b.x_start;
+  F.yv[0] = F.yv[4] = b.y_start;
+  F.xv[1] = b.x_start - b.width * std::sin(rot);
+  F.yv[1] = b.y_start + b.width * std::cos(rot);
+  F.xv[2] = F.xv[1] + b.length * std::cos(rot);
+  F.yv[2] = F.yv[1] + b.length * std::sin(rot);
+  F.xv[3] = b.x_start + b.length * std::cos(rot);
+  F.yv[3] = b.y_start + b.length * std::sin(rot);
+  float x_min = F.xv[0], x_max = F.xv[0], y_min = F.yv[0], y_max = F.yv[0];
+  for (int v = 1; v < 5; ++v) {
+    x_min = F.xv[v] < x_min ? F.xv[v] : x_min;
+    x_max = F.xv[v] > x_max ? F.xv[v] : x_max;
+    y_min = F.yv[v] < y_min ? F.yv[v] : y_min;
+    y_max = F.yv[v] > y_max ? F.yv[v] : y_max;
+  }
+  F.i0 = (int)(x_min / dx);
+  F.i1 = (int)((x_max / dx) + 1);
+  F.j0 = (int)(y_min / dy);
+  F.j1 = (int)((y_max / dy) + 1);
+  const int zn = nz - 1;
+  F.k0 = F.k1 = 1;
+  for (int k = 1; k < zn; ++k) {
+    F.k0 = k;
+    if (b.base_height <= z_face[k + 1]) break;
+  }
+  for (int k = 0; k < zn; ++k) {
+    F.k1 = k + 1;
+    if (k + 1 >= zn || height_eff < z[k + 1]) break;
+  }
+  return F;
+}
+
+int inflow_profile(int nz, const std::vector<float> &z, const qesmc_inflow &in, std::vector<float> &up,
+                   std::vector<float> &vp)
+{
+  up.assign(nz, 0.f);
+  vp.assign(nz, 0.f);
+  const float vk = 0.4f;
+  const float theta = (float)((270.0 - in.wind_dir_deg) * M_PI / 180.0);
+  const float ct = std::cos(theta), st = std::sin(theta);
+  const float U = in.U_ref, zr = in.z_ref, zp = in.z0_or_p;
+  if (in.kind == 0) {
+    for (int k = 1; k < nz; ++k) {
+      up[k] = ct * U;
+      vp[k] = st * U;
+    }
+  } else if (in.kind == 1) {
+    const float psi = 0.0f;  // neutral stability
+    const float u_star = U * vk / (std::log((zr + zp) / zp) + psi);
+    for (int k = 1; k < nz - 1; ++k) {
+      const float lg = std::log(((z[k] - 0.0f) + zp) / zp) + psi;
+      up[k] = (ct * u_star / vk) * lg;
+      vp[k] = (st * u_star / vk) * lg;
+      if (std::fabs(u_star) >= 2 * U) {
+        up[k] = ct * U;
+        vp[k] = st * U;
+      }
+    }
+  } else if (in.kind == 2) {
+    for (int k = 1; k < nz - 1; ++k) {
+      const float pw = std::pow(((z[k] - 0.0f) / zr), zp);
+      up[k] = ct * U * pw;
+      vp[k] = st * U * pw;
+    }
+  } else {
+    return QESMC_ERR_ARG;
+  }
+  return QESMC_OK;
+}
+
+}  // namespace qesmc
+
+namespace {
+
+using qesmc::is_solid_flag;
+
+struct Grid {
+  int nx, ny, nz;
+  long long sy, sz, fy, fz;
+  long long cell(int i, int j, int k) const { return i + j * sy + k * sz; }
+  long long face(int i, int j, int k) const { return i + j * fy + k * fz; }
+};
+
 // PolyBuilding::setCellFlags for one rectangle (meshTypeFlag 0).
 void stamp_building(const Grid &G, float dx, const std::vector<float> &z_face,
                     const std::vector<float> &z, const qesmc_building &b, int *flag)
 {
-  const float dy = dx;  // getDomainSize() returns (dx, dx, dz), Domain.h:104
-  const float height_eff = b.height + b.base_height;
-  const float rot = (float)(b.rotation_deg * M_PI / 180.0);
-  float xv[5], yv[5];
-  xv[0] = xv[4] = b.x_start;
-  yv[0] = yv[4] = b.y_start;
-  xv[1] = b.x_start - b.width * std::sin(rot);
-  yv[1] = b.y_start + b.width * std::cos(rot);
-  xv[2] = xv[1] + b.length * std::cos(rot);
-  yv[2] = yv[1] + b.length * std::sin(rot);
-  xv[3] = b.x_start + b.length * std::cos(rot);
-  yv[3] = b.y_start + b.length * std::sin(rot);
-  float x_min = xv[0], x_max = xv[0], y_min = yv[0], y_max = yv[0];
-  for (int v = 1; v < 5; ++v) {
-    x_min = xv[v] < x_min ? xv[v] : x_min;
-    x_max = xv[v] > x_max ? xv[v] : x_max;
-    y_min = yv[v] < y_min ? yv[v] : y_min;
-    y_max = yv[v] > y_max ? yv[v] : y_max;
-  }
-  const int i0 = (int)(x_min / dx), i1 = (int)((x_max / dx) + 1);
-  const int j0 = (int)(y_min / dy), j1 = (int)((y_max / dy) + 1);
-  const int zn = G.nz - 1;
-  int k0 = 1, k1 = 1;
-  for (int k = 1; k < zn; ++k) {
-    k0 = k;
-    if (b.base_height <= z_face[k + 1]) break;
-  }
-  for (int k = 0; k < zn; ++k) {
-    k1 = k + 1;
-    if (k + 1 >= zn || height_eff < z[k + 1]) break;
-  }
-  for (int j = j0; j <= j1; ++j) {
-    const float yc = (j + 0.5) * dy;
-    for (int i = i0; i <= i1; ++i) {
+  const qesmc::Footprint F = qesmc::building_footprint(G.nz, dx, z_face, z, b);
+  for (int j = F.j0; j <= F.j1; ++j) {
+    const float yc = (j + 0.5) * dx;
+    for (int i = F.i0; i <= F.i1; ++i) {
       const float xc = (i + 0.5) * dx;
-      // crossing-number test at the cell centre (closed ring of 5 vertices)
-      int crossings = 0;
-      unsigned v = 0, start = 0;
-      while (v < 4) {
-        const bool straddles = (yv[v] <= yc && yv[v + 1] > yc) || (yv[v] > yc && yv[v + 1] <= yc);
-        if (straddles) {
-          const float t = (yc - yv[v]) / (yv[v + 1] - yv[v]);
-          if (xc < (xv[v] + t * (xv[v + 1] - xv[v]))) ++crossings;
-        }
-        ++v;
-        if (xv[v] == xv[start] && yv[v] == yv[start]) {
-          ++v;
-          start = v;
-        }
-      }
-      if ((crossings & 1) == 0) continue;
+      if (!qesmc::footprint_contains(F, xc, yc)) continue;
       if (i < 0 || j < 0 || i >= G.nx - 1 || j >= G.ny - 1) continue;
-      for (int k = k0; k < k1; ++k) {
+      for (int k = F.k0; k < F.k1; ++k) {
         int &fl = flag[G.cell(i, j, k)];
         if (fl != 7) fl = 0;
       }
@@ -138,7 +173,7 @@ int qesmc_city_build(int nx, int ny, int nz, float dx, const float *dz_array, in
   const long long ncell = (long long)(nx - 1) * (ny - 1) * (nz - 1);
   const long long nface = (long long)nx * ny * nz;
   std::vector<float> z_face, z;
-  vertical_grid(nz, dz_array, z_face, z);
+  qesmc::vertical_grid(nz, dz_array, z_face, z);
 
   // allocateMemory: coefficients 1, cells fluid; ground ghost layer terrain
   float *coef[6] = {e, f, g, h, m, n};
@@ -167,13 +202,13 @@ int qesmc_city_build(int nx, int ny, int nz, float dx, const float *dz_array, in
     for (int j = 0; j < ny - 1; ++j)
       for (int i = 0; i < nx - 1; ++i) {
         const long long c = G.cell(i, j, k);
-        if (is_solid(flag[c])) continue;
-        if (is_solid(flag[c - G.sz])) n[c] = 0.0f;
-        if (is_solid(flag[c + G.sz])) m[c] = 0.0f;
-        if (i > 0 && is_solid(flag[c - 1])) f[c] = 0.0f;
-        if (is_solid(flag[c + 1])) e[c] = 0.0f;
-        if (j > 0 && is_solid(flag[c - G.sy])) h[c] = 0.0f;
-        if (is_solid(flag[c + G.sy])) g[c] = 0.0f;
+        if (is_solid_flag(flag[c])) continue;
+        if (is_solid_flag(flag[c - G.sz])) n[c] = 0.0f;
+        if (is_solid_flag(flag[c + G.sz])) m[c] = 0.0f;
+        if (i > 0 && is_solid_flag(flag[c - 1])) f[c] = 0.0f;
+        if (is_solid_flag(flag[c + 1])) e[c] = 0.0f;
+        if (j > 0 && is_solid_flag(flag[c - G.sy])) h[c] = 0.0f;
+        if (is_solid_flag(flag[c + G.sy])) g[c] = 0.0f;
       }
 
   // Wall::solverCoefficients (dy taken as dx, Domain.h:104)
@@ -194,37 +229,8 @@ int qesmc_city_build(int nx, int ny, int nz, float dx, const float *dz_array, in
   }
 
   // sensor profile on flat ground (z_terrain = 0, neutral: psi = 0)
-  std::vector<float> up(nz, 0.f), vp(nz, 0.f);
-  const float vk = 0.4f;
-  const float theta = (float)((270.0 - inflow->wind_dir_deg) * M_PI / 180.0);
-  const float ct = std::cos(theta), st = std::sin(theta);
-  const float U = inflow->U_ref, zr = inflow->z_ref, zp = inflow->z0_or_p;
-  if (inflow->kind == 0) {
-    for (int k = 1; k < nz; ++k) {
-      up[k] = ct * U;
-      vp[k] = st * U;
-    }
-  } else if (inflow->kind == 1) {
-    const float psi = 0.0f;
-    const float u_star = U * vk / (std::log((zr + zp) / zp) + psi);
-    for (int k = 1; k < nz - 1; ++k) {
-      const float lg = std::log(((z[k] - 0.0f) + zp) / zp) + psi;
-      up[k] = (ct * u_star / vk) * lg;
-      vp[k] = (st * u_star / vk) * lg;
-      if (std::fabs(u_star) >= 2 * U) {
-        up[k] = ct * U;
-        vp[k] = st * U;
-      }
-    }
-  } else if (inflow->kind == 2) {
-    for (int k = 1; k < nz - 1; ++k) {
-      const float pw = std::pow(((z[k] - 0.0f) / zr), zp);
-      up[k] = ct * U * pw;
-      vp[k] = st * U * pw;
-    }
-  } else {
-    return QESMC_ERR_ARG;
-  }
+  std::vector<float> up, vp;
+  if (qesmc::inflow_profile(nz, z, *inflow, up, vp) != QESMC_OK) return QESMC_ERR_ARG;
   std::memset(u0, 0, sizeof(float) * nface);
   std::memset(v0, 0, sizeof(float) * nface);
   std::memset(w0, 0, sizeof(float) * nface);
@@ -240,7 +246,7 @@ int qesmc_city_build(int nx, int ny, int nz, float dx, const float *dz_array, in
   for (int k = 0; k < nz - 1; ++k)
     for (int j = 1; j < ny - 1; ++j)
       for (int i = 1; i < nx - 1; ++i) {
-        if (!is_solid(flag[G.cell(i, j, k)])) continue;
+        if (!is_solid_flag(flag[G.cell(i, j, k)])) continue;
         const long long fc = G.face(i, j, k);
         u0[fc] = u0[fc + 1] = 0.0f;
         v0[fc] = v0[fc + G.fy] = 0.0f;

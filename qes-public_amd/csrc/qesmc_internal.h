@@ -95,4 +95,21 @@ int run_prepare(qesmc_ctx *x, const qesmc_params *pp, RunState &st);
 int run_finish(qesmc_ctx *x, const RunState &st, int launched, qesmc_result *res);
 float row_max(const unsigned *row);
 
+// Geometry set-up shared by qesmc_set_geometry and the device city builder:
+// reserve (re)allocates the device arrays when the grid size changes;
+// commit encodes the stencil of the resident natural coefficients.
+int geometry_reserve(qesmc_ctx *x, int nx, int ny, int nz);
+int geometry_commit(qesmc_ctx *x);
+
+inline int map_hip_error(hipError_t e)
+{
+  switch (e) {
+  case hipSuccess: return QESMC_OK;
+  case hipErrorOutOfMemory: return QESMC_ERR_OOM;
+  case hipErrorNoDevice:
+  case hipErrorInvalidDevice: return QESMC_ERR_NO_DEVICE;
+  default: return QESMC_ERR_HIP;
+  }
+}
+
 }  // namespace qesmc

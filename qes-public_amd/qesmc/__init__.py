@@ -190,21 +190,30 @@ class City:
                     n=self.n, icellflag=self.icellflag, u0=self.u0, v0=self.v0, w0=self.w0)
 
 
-def city_build(ncx, ncy, dx, dz_array, buildings=(), inflow=(1, 5.0, 10.0, 0.1, 270.0)) -> City:
-    """Flat stair-step city through qesmc_city_build."""
-    dz_array = np.ascontiguousarray(dz_array, np.float32)
-    nx, ny, nz = ncx + 1, ncy + 1, dz_array.size + 1
-    ncell, nface = (nx - 1) * (ny - 1) * (nz - 1), nx * ny * nz
+def _buildings(buildings):
     bl = list(buildings)
     barr = (Building * max(1, len(bl)))()
     for q, b in enumerate(bl):
         vals = (list(b) + [0.0, 0.0])[:7] if not isinstance(b, Building) else None
         barr[q] = b if vals is None else Building(*[float(v) for v in vals])
-    inf = Inflow(int(inflow[0]), *[float(v) for v in inflow[1:5]])
+    return barr, len(bl)
+
+
+def _inflow(inflow):
+    return Inflow(int(inflow[0]), *[float(v) for v in inflow[1:5]])
+
+
+def city_build(ncx, ncy, dx, dz_array, buildings=(), inflow=(1, 5.0, 10.0, 0.1, 270.0)) -> City:
+    """Flat stair-step city through qesmc_city_build."""
+    dz_array = np.ascontiguousarray(dz_array, np.float32)
+    nx, ny, nz = ncx + 1, ncy + 1, dz_array.size + 1
+    ncell, nface = (nx - 1) * (ny - 1) * (nz - 1), nx * ny * nz
+    barr, nb = _buildings(buildings)
+    inf = _inflow(inflow)
     co = [np.empty(ncell, np.float32) for _ in range(6)]
     flag = np.empty(ncell, np.int32)
     u0, v0, w0 = (np.empty(nface, np.float32) for _ in range(3))
-    rc = lib().qesmc_city_build(nx, ny, nz, float(dx), _f(dz_array), len(bl), barr, C.byref(inf),
+    rc = lib().qesmc_city_build(nx, ny, nz, float(dx), _f(dz_array), nb, barr, C.byref(inf),
                                 *[_f(a) for a in co], _i(flag), _f(u0), _f(v0), _f(w0))
     _check(rc, "qesmc_city_build")
     return City(nx, ny, nz, dx, dx, dz_array, *co, flag, u0, v0, w0)
@@ -302,6 +311,32 @@ class Context:
                "qesmc_divergence_linf")
         return out.value
 
+    def set_city(self, ncx, ncy, dx, dz_array, buildings=(), inflow=(1, 5.0, 10.0, 0.1, 270.0)):
+        """Build a flat stair-step city on the device (qesmc_set_city)."""
+        dz_array = np.ascontiguousarray(dz_array, np.float32)
+        barr, nb = _buildings(buildings)
+        inf = _inflow(inflow)
+        nx, ny, nz = ncx + 1, ncy + 1, dz_array.size + 1
+        _check(_city_lib().qesmc_set_city(self._h, nx, ny, nz, float(dx), _f(dz_array), nb, barr, C.byref(inf)),
+               "qesmc_set_city")
+        self.city = CityShape(nx, ny, nz, dx, dx, dz_array)
+        return self.city
+
+    def set_city_inflow(self, inflow):
+        """New sensor profile on the resident geometry (qesmc_set_city_inflow)."""
+        inf = _inflow(inflow)
+        _check(_city_lib().qesmc_set_city_inflow(self._h, C.byref(inf)), "qesmc_set_city_inflow")
+
+    def get_inputs(self) -> City:
+        """The resident inputs as a City (qesmc_get_inputs)."""
+        s = self.city
+        co = [np.empty(s.ncell, np.float32) for _ in range(6)]
+        flag = np.empty(s.ncell, np.int32)
+        u0, v0, w0 = (np.empty(s.nface, np.float32) for _ in range(3))
+        _check(_city_lib().qesmc_get_inputs(self._h, *[_f(a) for a in co], _i(flag), _f(u0), _f(v0), _f(w0)),
+               "qesmc_get_inputs")
+        return City(s.nx, s.ny, s.nz, s.dx, s.dy, s.dz_array, *co, flag, u0, v0, w0)
+
     def solve(self, city: City, lam=None, params: Params | None = None, **kw):
         """One-call drop-in (qesmc_solve): returns (u, v, w, lambda, result)."""
         p = params if params is not None else default_params(**kw)
@@ -350,6 +385,36 @@ def autotune(ctx: "Context", iterations=40, params: Params | None = None, **kw) 
     best = Params()
     _check(_batch_lib().qesmc_autotune(ctx._h, C.byref(p), int(iterations), C.byref(best)), "qesmc_autotune")
     return best
+
+
+EXPORTS = EXPORTS + ("qesmc_set_city", "qesmc_set_city_inflow", "qesmc_get_inputs")
+
+
+class CityShape:
+    """Sizes of a city that lives only on the device."""
+
+    def __init__(self, nx, ny, nz, dx, dy, dz_array):
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.dx, self.dy = float(dx), float(dy)
+        self.dz_array = np.ascontiguousarray(dz_array, np.float32)
+
+    ncell = City.ncell
+    nface = City.nface
+    interior_lups = City.interior_lups
+
+
+def _city_lib():
+    L = lib()
+    if not getattr(L, "_city_sigs_set", False):
+        L.qesmc_set_city.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_float, _fp, C.c_int,
+                                     C.POINTER(Building), C.POINTER(Inflow)]
+        L.qesmc_set_city.restype = C.c_int
+        L.qesmc_set_city_inflow.argtypes = [C.c_void_p, C.POINTER(Inflow)]
+        L.qesmc_set_city_inflow.restype = C.c_int
+        L.qesmc_get_inputs.argtypes = [C.c_void_p] + [_fp] * 6 + [_ip] + [_fp] * 3
+        L.qesmc_get_inputs.restype = C.c_int
+        L._city_sigs_set = True
+    return L
 
 
 UNET_IN_CHANNELS = 8
