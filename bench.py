@@ -8,6 +8,19 @@ solid zeroing, with every input already resident in HBM.  `value` is GLUPS
 
   python bench.py [--steps K] [--warmup W] [--gpus 1]
 
+Further legs reported in the same JSON line:
+  time_to_tol  128 x 128 x 64 held-out cities (the warm-start workload):
+               iterations and device ms until the velocity field is within
+               1e-4 m/s (L_inf) of the FP64 converged oracle, cold start
+               versus the trained U-Net's lambda0; U-Net MFMA throughput.
+  batch        64 cities of uneven size (64-256 x 64-256 x 64-96 cells),
+               wind directions 5.625 deg apart, 500 SOR iterations each:
+               one batched launch per colour pass (HIP graph replay) versus
+               the instances solved one after another.
+  cpu_baseline the oracle's fp32 red-black restatement on the host cores
+               (bounded sample); the same leg runs the FP64 oracle that the
+               time_to_tol leg checks the GPU fields against.
+
 Prints one JSON line.  --gpus above 1 is refused: this build runs on one GPU.
 """
 import argparse
@@ -56,13 +69,29 @@ def bytes_per_lup(res):
     return 17.0 if res["compressed"] else 40.0
 
 
+def _oracle():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    return O
+
+
+def fp64_checker(cities):
+    """FP64 converged velocity fields (oracle red-black SOR to max|dlambda| <=
+    1e-9 in double) of the given cities: the checker of the time_to_tol leg."""
+    O = _oracle()
+    out = {}
+    for seed, city in cities.items():
+        u, v, w, _, its, _ = O.solve_rb_f64(O.Problem(**city.arrays()), tol=1e-9)
+        out[seed] = (u, v, w, its)
+    return out
+
+
 def cpu_baseline(city, budget_s):
     """The oracle's fp32 red-black restatement of Solver_CPU_RB on the host
     cores: R once, then chunks of 10 SOR iterations until `budget_s` has
     elapsed (a bounded sample of the 500-iteration schedule)."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
-
+    O = _oracle()
     p = O.Problem(**city.arrays())
     t0 = time.perf_counter()
     R = O.divergence_f32(p)
@@ -77,6 +106,259 @@ def cpu_baseline(city, budget_s):
     return {"value": round(glups, 4), "unit": "GLUPS", "cores": O.num_threads(), "kind": "port",
             "sample": f"config2 domain, R + {its} of 500 SOR iterations ({el:.1f} s), oracle/qesmc_oracle.c "
                       f"Solver_CPU_RB restatement, OpenMP"}
+
+
+WEIGHTS = os.path.join(ROOT, "qes-public_amd", "qesmc", "weights", "unet_c4.npz")
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def _linf(a, b):
+    return float(np.abs(a.astype(np.float64) - b).max())
+
+
+def time_to_tol_leg(seeds, checker, tol_ms=1e-4, chunk=25, cap=6000):
+    """Cold versus U-Net warm start on held-out 128 x 128 x 64 cities.
+
+    Pass 1 (untimed) runs SOR in chunks of `chunk` iterations and reads the
+    velocity back after each, to find the first checkpoint whose L_inf error
+    against the FP64 field is <= tol_ms (and the error after the reference's
+    500 iterations).  Pass 2 times that many iterations from a fresh start:
+    R + SOR + correction on the device (qesmc_result.ms_total), plus the
+    U-Net inference for the warm start.  Without a checker the stop rule
+    max|dlambda| <= 1e-4 is used instead."""
+    from qesmc import warmstart as W
+
+    if not os.path.exists(WEIGHTS):
+        return {"skipped": "trained weights not found: " + WEIGHTS}
+    net = W.load(WEIGHTS)
+    rows = []
+    with Q.Context(0) as ctx:
+        for seed in seeds:
+            city = W.c4_city(seed)
+            W.load_problem(ctx, city)
+            row = {"seed": seed}
+            for mode in ("cold", "warm"):
+                ctx.set_lambda(None)
+                if mode == "warm":
+                    net.predict(ctx)
+                it, it_tol, err500, err_tol = 0, None, None, None
+                if checker is not None:
+                    u64, v64, w64, _ = checker[seed]
+                    while it < cap:
+                        ctx.run(itermax=chunk, tol=-1.0)
+                        it += chunk
+                        u, v, w = ctx.get_velocity()
+                        err = max(_linf(u, u64), _linf(v, v64), _linf(w, w64))
+                        if it == 500:
+                            err500 = err
+                        if it_tol is None and err <= tol_ms:
+                            it_tol, err_tol = it, err
+                        if it_tol is not None and it >= 500:
+                            break
+                # pass 2: timed from a fresh start
+                ctx.set_lambda(None)
+                unet_ms, tflops = net.predict(ctx) if mode == "warm" else (0.0, 0.0)
+                if checker is not None and it_tol is not None:
+                    r = ctx.run(itermax=it_tol, tol=-1.0)
+                else:
+                    r = ctx.run(itermax=20000, tol=1e-4)
+                row[mode] = {"iterations": r["iterations"], "ms": round(r["ms_total"] + unet_ms, 3),
+                             "sor_ms": round(r["ms_sor"], 3), "err_at_tol": err_tol, "err_at_500": err500}
+                if mode == "warm":
+                    row[mode].update(unet_ms=round(unet_ms, 3), unet_tflops=round(tflops, 1))
+            rows.append(row)
+    net.close()
+
+    def mean(mode, key):
+        vals = [r[mode][key] for r in rows if r[mode].get(key) is not None]
+        return float(np.mean(vals)) if vals else None
+
+    cold_ms, warm_ms = mean("cold", "ms"), mean("warm", "ms")
+    tfl = mean("warm", "unet_tflops")
+    return {
+        "workload": "%d held-out seeded cities, 128x128x64 cells, dx=2 m, dz 1 m x1.02, 12-30 buildings, log law, "
+                    "random direction; U-Net 4 levels 16/32/64/128 trained on 2000 solver-generated cities" % len(rows),
+        "criterion": ("L_inf(u,v,w) vs FP64 oracle <= %g m/s, checked every %d iterations" % (tol_ms, chunk)
+                      if checker is not None else "stop rule max|dlambda| <= 1e-4 (no FP64 checker)"),
+        "cold": {"iterations": mean("cold", "iterations"), "ms": cold_ms, "err_at_500": mean("cold", "err_at_500")},
+        "warm": {"iterations": mean("warm", "iterations"), "ms": warm_ms, "err_at_500": mean("warm", "err_at_500"),
+                 "unet_ms": mean("warm", "unet_ms")},
+        "speedup_warm_vs_cold": round(cold_ms / warm_ms, 3) if cold_ms and warm_ms else None,
+        "unet_mfma": {"tflops": tfl, "peak_bf16_dense": BF16_PEAK_TFLOPS,
+                      "frac": round(tfl / BF16_PEAK_TFLOPS, 4) if tfl else None},
+        "per_city": rows,
+    }
+
+
+def batch_cities(n=64, seed=3):
+    """Uneven ensemble: (ncx, ncy) uniform in 64..256, ncz in 64..96, 20-120
+    requested buildings (footprints U(10,40) m, heights U(10,60) m; fewer are
+    placed where they do not fit), dx = 2 m, dz 1 m x1.02, log law u* = 0.5 m/s
+    z0 = 0.5 m, wind directions q * 5.625 deg."""
+    rng = np.random.default_rng(seed)
+    u_ref = 0.5 / 0.4 * math.log((10.0 + 0.5) / 0.5)
+    out = []
+    for q in range(n):
+        ncx, ncy = (int(v) for v in rng.integers(64, 257, 2))
+        ncz = int(rng.integers(64, 97))
+        nb = int(rng.integers(20, 121))
+        blds = Q.random_buildings(5000 + q, ncx, ncy, 2.0, nb, 10.0, 40.0, 10.0, 60.0, margin=2)
+        out.append((ncx, ncy, Q.dz_geometric(ncz, 1.0, 1.02), blds, (1, u_ref, 10.0, 0.5, q * 5.625)))
+    return out
+
+
+def batch_leg(n=64, itermax=500, reps=2):
+    specs = batch_cities(n)
+    ctxs = [Q.Context(0) for _ in specs]
+    try:
+        t0 = time.perf_counter()
+        for c, (ncx, ncy, dz, blds, inflow) in zip(ctxs, specs):
+            c.set_city(ncx, ncy, 2.0, dz, blds, inflow)
+        build_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for c in ctxs:
+            Q.autotune(c, iterations=20)
+        tune_s = time.perf_counter() - t0
+        lups = sum(c.city.interior_lups() for c in ctxs) * itermax
+        best = {}
+        for g in (True, False):
+            ms = []
+            for _ in range(reps):
+                for c in ctxs:
+                    c.set_lambda(None)
+                res, ms_loop = Q.run_batch(ctxs, itermax=itermax, tol=1e-9, check_every=50, use_graph=g)
+                ms.append(ms_loop)
+            best["graph" if g else "eager"] = min(ms)
+        seq = []
+        for _ in range(reps):
+            tot = 0.0
+            for c in ctxs:
+                c.set_lambda(None)
+                tot += c.run(itermax=itermax, tol=1e-9)["ms_sor"]
+            seq.append(tot)
+        seq_ms = min(seq)
+    finally:
+        for c in ctxs:
+            c.close()
+    return {
+        "workload": "%d cities, 64-256 x 64-256 x 64-96 cells (%.1f M cells in total), %d buildings placed, "
+                    "directions 5.625 deg apart, %d SOR iterations each"
+                    % (n, sum(c[2].size * c[0] * c[1] for c in specs) / 1e6, sum(len(c[3]) for c in specs), itermax),
+        "batched_graph_ms": round(best["graph"], 3), "batched_eager_ms": round(best["eager"], 3),
+        "sequential_ms": round(seq_ms, 3),
+        "glups_batched": round(lups / best["graph"] / 1e6, 2), "glups_sequential": round(lups / seq_ms / 1e6, 2),
+        "speedup_batched_vs_sequential": round(seq_ms / best["graph"], 3),
+        "device_city_build_s": round(build_s, 3), "autotune_s": round(tune_s, 3),
+    }
+
+
+def large_city(dirs=8):
+    """1000 x 1000 x 150 cells, dx = dy = 1 m, dz geometric from 1 m at ratio
+    1.02 (top at 925 m), 3000 seeded buildings with heights lognormal (median
+    20 m, sigma 0.6) capped at 120 m on footprints U(6,16) m (3000 buildings
+    of the 400 x 400 city's 10-40 m footprints do not fit on 1 km^2), log law
+    u* = 0.5 m/s z0 = 0.5 m, `dirs` wind directions 360/dirs deg apart."""
+    ncx = ncy = 1000
+    dz = Q.dz_geometric(150, 1.0, 1.02)
+    blds = Q.random_buildings(555, ncx, ncy, 1.0, 3000, 6.0, 16.0, 20.0, 0.6, lognormal=True, hcap=120.0, margin=2)
+    u_ref = 0.5 / 0.4 * math.log((10.0 + 0.5) / 0.5)
+    winds = [q * 360.0 / dirs for q in range(dirs)]
+    return ncx, ncy, dz, blds, u_ref, winds
+
+
+def large_leg(dirs, tol, itermax, cpu_iters, unet):
+    """The 8-direction ensemble on the 1000 x 1000 x 150 city, every
+    direction solved to the stop rule max|dlambda| <= tol:
+      cold        the directions batched, each from lambda = 0;
+      superposed  two basis solves (wind along +x and +y, batched), then every
+                  direction batched from cos(theta) lambda_x + sin(theta) lambda_y;
+    plus the classic-layout path's time per iteration (the reference GPU
+    algorithm) and a CPU sample of the oracle (cpu_iters iterations)."""
+    ncx, ncy, dz, blds, u_ref, winds = large_city(dirs)
+    params = Q.default_params(itermax=itermax, tol=tol, check_every=200)
+    out = {"workload": "1000x1000x150 cells (151 M), dx=dy=1 m, dz 1 m x1.02, %d lognormal buildings "
+                       "(median 20 m, sigma 0.6, cap 120 m, footprints U(6,16) m), log law u*=0.5 m/s z0=0.5 m, "
+                       "%d directions %.0f deg apart; stop rule max|dlambda| <= %g (itermax %d)"
+                       % (len(blds), dirs, 360.0 / dirs, tol, itermax)}
+    ctxs = [Q.Context(0) for _ in range(dirs)]
+    basis = [Q.Context(0) for _ in range(2)]
+    try:
+        t0 = time.perf_counter()
+        for c, wd in zip(ctxs, winds):
+            c.set_city(ncx, ncy, 1.0, dz, blds, (1, u_ref, 10.0, 0.5, wd))
+        for c, wd in zip(basis, (270.0, 180.0)):
+            c.set_city(ncx, ncy, 1.0, dz, blds, (1, u_ref, 10.0, 0.5, wd))
+        out["device_city_build_s_per_context"] = round((time.perf_counter() - t0) / (dirs + 2), 3)
+        lups = ctxs[0].city.interior_lups()
+        # classic layout, reference GPU algorithm: time per iteration
+        r = ctxs[0].run(itermax=100, tol=-1.0, path=Q.PATH_PLAIN)
+        plain_ms_it = r["ms_sor"] / 100
+        r = ctxs[0].run(itermax=100, tol=-1.0)
+        split_ms_it = r["ms_sor"] / 100
+        out["ms_per_iteration"] = {"plain": round(plain_ms_it, 4), "split": round(split_ms_it, 4)}
+        out["glups_single"] = {"plain": round(lups / plain_ms_it / 1e6, 1), "split": round(lups / split_ms_it / 1e6, 1)}
+        print(f"large: plain {plain_ms_it:.3f} ms/it, split {split_ms_it:.3f} ms/it", file=sys.stderr, flush=True)
+        # cold: all directions batched from zero
+        for c in ctxs:
+            c.set_lambda(None)
+        t0 = time.perf_counter()
+        res, ms = Q.run_batch(ctxs, params)
+        cold_wall = time.perf_counter() - t0
+        cold_its = [r["iterations"] for r in res]
+        cold_uvw = [c.get_velocity() for c in ctxs]  # host copies for the comparison below
+        print(f"large: cold {ms:.0f} ms, iterations {cold_its}", file=sys.stderr, flush=True)
+        out["cold"] = {"ms": round(ms, 1), "wall_s": round(cold_wall, 2), "iterations": cold_its,
+                       "final_max_dlambda": [r["final_max_dlambda"] for r in res],
+                       "plain_path_ms_extrapolated": round(plain_ms_it * sum(cold_its), 1)}
+        for c in basis:
+            c.set_lambda(None)
+        unet_ms = 0.0
+        if unet is not None:
+            for c in basis:
+                ms_u, _ = unet.predict(c)
+                unet_ms += ms_u
+        t0 = time.perf_counter()
+        rb, ms_b = Q.run_batch(basis, params)
+        tc = time.perf_counter()
+        for c, wd in zip(ctxs, winds):
+            th = math.radians(270.0 - wd)
+            Q.lambda_combine(c, basis, np.array([math.cos(th), math.sin(th)], np.float32))
+        combine_ms = (time.perf_counter() - tc) * 1e3
+        rw, ms_w = Q.run_batch(ctxs, params)
+        sup_wall = time.perf_counter() - t0
+        diffs = []
+        for c, ref in zip(ctxs, cold_uvw):
+            diffs.append(max(float(np.abs(a - b).max()) for a, b in zip(c.get_velocity(), ref)))
+        del cold_uvw
+        sup_ms = ms_b + combine_ms + ms_w + unet_ms
+        print(f"large: superposed basis {ms_b:.0f} ms {[r['iterations'] for r in rb]}, directions {ms_w:.0f} ms "
+              f"{[r['iterations'] for r in rw]}", file=sys.stderr, flush=True)
+        out["superposed"] = {"ms": round(sup_ms, 1), "wall_s": round(sup_wall, 2), "basis_ms": round(ms_b, 1),
+                             "basis_iterations": [r["iterations"] for r in rb], "combine_ms": round(combine_ms, 1),
+                             "directions_ms": round(ms_w, 1), "direction_iterations": [r["iterations"] for r in rw],
+                             "unet_ms": round(unet_ms, 1) if unet is not None else None,
+                             "linf_vs_cold_m_s": diffs}
+        out["speedup_superposed_vs_cold"] = round(out["cold"]["ms"] / sup_ms, 2)
+        out["speedup_superposed_vs_plain_cold"] = round(out["cold"]["plain_path_ms_extrapolated"] / sup_ms, 2)
+        if cpu_iters > 0:
+            city = ctxs[0].get_inputs()
+            O = _oracle()
+            p = O.Problem(**city.arrays())
+            t0 = time.perf_counter()
+            R = O.divergence_f32(p)
+            O.sor_rb_f32(p, R, None, itermax=cpu_iters, tol=-1.0)
+            el = time.perf_counter() - t0
+            cpu_ms_it = el * 1e3 / cpu_iters
+            out["cpu_baseline"] = {"ms_per_iteration": round(cpu_ms_it, 2), "cores": O.num_threads(), "kind": "port",
+                                   "glups": round(lups / cpu_ms_it / 1e6, 3),
+                                   "sample": "R + %d SOR iterations of direction 0 (%.1f s), extrapolated to the "
+                                             "cold iteration counts" % (cpu_iters, el),
+                                   "ms_extrapolated": round(cpu_ms_it * sum(cold_its), 0)}
+            out["speedup_superposed_vs_cpu"] = round(out["cpu_baseline"]["ms_extrapolated"] / sup_ms, 1)
+    finally:
+        for c in ctxs + basis:
+            c.close()
+    return out
 
 
 def load_traffic(workload_key):
@@ -99,6 +381,15 @@ def main():
     ap.add_argument("--tile-v", type=int, default=0, help="split path: half-cells per lane (0: library choice)")
     ap.add_argument("--wg-per-cu", type=int, default=0, help="split path: work-groups per CU (0: library choice)")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline sampling (0: skip)")
+    ap.add_argument("--ttol-cities", type=int, default=2, help="time_to_tol leg: held-out cities (0: skip)")
+    ap.add_argument("--batch", type=int, default=64, help="batch leg: instances (0: skip)")
+    ap.add_argument("--workload", choices=["config2", "large"], default="config2",
+                    help="large: only the 1000x1000x150 8-direction ensemble (minutes of GPU time)")
+    ap.add_argument("--large-dirs", type=int, default=8)
+    ap.add_argument("--large-tol", type=float, default=1e-3, help="stop rule of the large workload")
+    ap.add_argument("--large-itermax", type=int, default=200000)
+    ap.add_argument("--large-cpu-iters", type=int, default=20)
+    ap.add_argument("--large-unet", action="store_true", help="U-Net lambda0 for the two basis solves")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run 5 plain device copies of this many MiB (HBM counter calibration under rocprofv3)")
     a = ap.parse_args()
@@ -120,6 +411,16 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": n * 4}))
+        return
+
+    if a.workload == "large":
+        unet = None
+        if a.large_unet:
+            from qesmc import warmstart as W
+
+            unet = W.load(WEIGHTS, size=(1000, 1000, 150))
+        out = large_leg(a.large_dirs, a.large_tol, a.large_itermax, a.large_cpu_iters, unet)
+        print(json.dumps({"metric": METRIC, "workload": "large", "n_gpus": 1, "large": out}))
         return
 
     city, desc = config2_city()
@@ -181,8 +482,20 @@ def main():
                      "avg_launch_us": round(launch_ms * 1e3, 2)},
         "cpu_baseline": None,
     }
+    checker = None
+    seeds = list(range(1_000_000, 1_000_000 + a.ttol_cities))
     if a.cpu_budget > 0:
         out["cpu_baseline"] = cpu_baseline(city, a.cpu_budget)
+        if seeds:
+            from qesmc import warmstart as W
+
+            checker = fp64_checker({s_: W.c4_city(s_) for s_ in seeds})
+    if seeds:
+        out["time_to_tol"] = time_to_tol_leg(seeds, checker)
+        if checker is not None:
+            out["time_to_tol"]["fp64_oracle_iterations"] = [checker[s_][3] for s_ in seeds]
+    if a.batch > 0:
+        out["batch"] = batch_leg(a.batch)
     print(json.dumps(out))
 
 

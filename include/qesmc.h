@@ -224,6 +224,21 @@ int qesmc_set_city_inflow(qesmc_ctx *ctx, const qesmc_inflow *inflow);
 int qesmc_get_inputs(qesmc_ctx *ctx, float *e, float *f, float *g, float *h, float *m, float *n,
                      int *icellflag, float *u0, float *v0, float *w0);
 
+/* ---- ensembles on one geometry (no counterpart in the reference) ----------
+ *
+ * The discrete problem is linear in (u0, v0, w0), and a single-sensor inflow
+ * from direction theta is cos(theta) x the inflow along x plus sin(theta) x
+ * the inflow along y.  Combining converged basis solutions gives an initial
+ * lambda for any direction; SOR then runs to the same stop rule. */
+
+/* lambda(ctx) = sum_t coefs[t] * lambda(srcs[t]), 1 <= n <= 8, all contexts on
+ * one device with the same grid (a source may be ctx itself). */
+int qesmc_lambda_combine(qesmc_ctx *ctx, int n, qesmc_ctx *const *srcs, const float *coefs);
+
+/* *out = max over all faces of |u_a - u_b|, |v_a - v_b|, |w_a - w_b| (FP64;
+ * a NaN difference gives +inf).  Contexts on one device, same grid. */
+int qesmc_velocity_linf_diff(qesmc_ctx *a, qesmc_ctx *b, double *out);
+
 /* ---- learned warm start: 3-D U-Net predicting lambda0 --------------------
  *
  * No counterpart in the reference (whose solvers start from lambda = 0 or the
@@ -281,8 +296,9 @@ int qesmc_unet_train(qesmc_unet *net, int steps, float lr, int first, int last,
 int qesmc_unet_eval(qesmc_unet *net, int first, int last, float *mean_loss);
 /* Predict lambda0 for the resident problem of ctx and install it as the
  * initial lambda of the next qesmc_run (boundary ring and top kept at 0, so
- * the converged solution is unchanged).  ms: device time of the inference;
- * tflops: achieved BF16 MFMA rate of its convolutions. */
+ * the converged solution is unchanged).  ms: wall time of the whole call
+ * (R and input features, network, installation of lambda0); tflops: achieved
+ * BF16 MFMA rate of the network's convolutions (device time of the network). */
 int qesmc_unet_predict(qesmc_unet *net, qesmc_ctx *ctx, double *ms, double *tflops);
 
 #ifdef __cplusplus

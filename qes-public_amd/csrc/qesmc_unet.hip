@@ -14,6 +14,7 @@
 // same kernel with tap-flipped, transposed weights; the weight gradient is an
 // MFMA GEMM reducing over voxels (split over work-groups, FP32 atomics).
 // FP32 master weights, Adam.
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -1158,6 +1159,8 @@ int qesmc_unet_predict(qesmc_unet *u, qesmc_ctx *x, double *ms, double *tflops)
   if (!u || !x) return QESMC_ERR_ARG;
   if (!(u->tscale > 0.f)) return QESMC_ERR_STATE;
   UCHK(hipSetDevice(u->device));
+  UCHK(hipStreamSynchronize(x->stream));
+  const auto t0 = std::chrono::steady_clock::now();
   int rc = problem_features(u, x);
   if (rc != QESMC_OK) return rc;
   hipStream_t s = u->stream;
@@ -1168,9 +1171,10 @@ int qesmc_unet_predict(qesmc_unet *u, qesmc_ctx *x, double *ms, double *tflops)
                      x->d_lam);
   UCHK(hipGetLastError());
   UCHK(hipStreamSynchronize(s));
+  const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   float t = 0.f;
   (void)hipEventElapsedTime(&t, u->ev[0], u->ev[1]);
-  if (ms) *ms = t;
+  if (ms) *ms = wall;
   if (tflops) *tflops = t > 0.f ? u->conv_flops / (t * 1e-3) / 1e12 : 0.0;
   return QESMC_OK;
 }

@@ -387,7 +387,8 @@ def autotune(ctx: "Context", iterations=40, params: Params | None = None, **kw) 
     return best
 
 
-EXPORTS = EXPORTS + ("qesmc_set_city", "qesmc_set_city_inflow", "qesmc_get_inputs")
+EXPORTS = EXPORTS + ("qesmc_set_city", "qesmc_set_city_inflow", "qesmc_get_inputs", "qesmc_lambda_combine",
+                     "qesmc_velocity_linf_diff")
 
 
 class CityShape:
@@ -413,8 +414,27 @@ def _city_lib():
         L.qesmc_set_city_inflow.restype = C.c_int
         L.qesmc_get_inputs.argtypes = [C.c_void_p] + [_fp] * 6 + [_ip] + [_fp] * 3
         L.qesmc_get_inputs.restype = C.c_int
+        L.qesmc_lambda_combine.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p), _fp]
+        L.qesmc_lambda_combine.restype = C.c_int
+        L.qesmc_velocity_linf_diff.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]
+        L.qesmc_velocity_linf_diff.restype = C.c_int
         L._city_sigs_set = True
     return L
+
+
+def lambda_combine(ctx: "Context", srcs, coefs):
+    """lambda(ctx) = sum coefs[t] * lambda(srcs[t]) on the device (qesmc_lambda_combine)."""
+    n = len(srcs)
+    hs = (C.c_void_p * n)(*[c._h.value for c in srcs])
+    cf = np.ascontiguousarray(coefs, np.float32)
+    _check(_city_lib().qesmc_lambda_combine(ctx._h, n, hs, _f(cf)), "qesmc_lambda_combine")
+
+
+def velocity_linf_diff(a: "Context", b: "Context") -> float:
+    """max |(u,v,w)_a - (u,v,w)_b| over all faces, FP64 on the device."""
+    out = C.c_double()
+    _check(_city_lib().qesmc_velocity_linf_diff(a._h, b._h, C.byref(out)), "qesmc_velocity_linf_diff")
+    return out.value
 
 
 UNET_IN_CHANNELS = 8

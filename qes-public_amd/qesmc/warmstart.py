@@ -79,9 +79,11 @@ def save(net: UNet, path: str, **meta):
                         size=np.array([net.ncx, net.ncy, net.ncz]), **meta)
 
 
-def load(path: str, device=0) -> UNet:
+def load(path: str, device=0, size=None) -> UNet:
+    """Trained network from `path`; `size` = (ncx, ncy, ncz) instantiates the
+    same (fully convolutional) weights for another domain size."""
     z = np.load(path)
-    ncx, ncy, ncz = (int(v) for v in z["size"])
+    ncx, ncy, ncz = (int(v) for v in (z["size"] if size is None else size))
     net = UNet(ncx, ncy, ncz, levels=int(z["levels"]), base_channels=int(z["base_channels"]), device=device,
                target_scale=float(z["target_scale"]))
     net.set_params(z["params"])

@@ -1,0 +1,82 @@
+"""What the trained U-Net warm start gains over the cold start, measured
+against the FP64 converged oracle on held-out 128 x 128 x 64 cities (seeds
+never used in training), with the committed weights."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+WEIGHTS = os.path.join(ROOT, "qes-public_amd", "qesmc", "weights", "unet_c4.npz")
+SEEDS = (1_000_000, 1_000_001, 1_000_002)
+CHUNK = 25
+
+
+def linf(ctx, ref):
+    u, v, w = ctx.get_velocity()
+    return max(float(np.abs(a.astype(np.float64) - b).max()) for a, b in zip((u, v, w), ref))
+
+
+def error_curve(ctx, ref, upto):
+    """L_inf velocity error against `ref` after every CHUNK iterations."""
+    out = []
+    for _ in range(upto // CHUNK):
+        ctx.run(itermax=CHUNK, tol=-1.0)
+        out.append(linf(ctx, ref))
+    return np.array(out)
+
+
+@pytest.fixture(scope="module")
+def cases(qesmc, oracle):
+    from qesmc import warmstart as W
+
+    out = []
+    for s in SEEDS:
+        city = W.c4_city(s)
+        u, v, w, _, _, _ = oracle.solve_rb_f64(oracle.Problem(**city.arrays()), tol=1e-9)
+        out.append((s, city, (u, v, w)))
+    return out
+
+
+def test_weights_are_the_trained_network():
+    z = np.load(WEIGHTS)
+    assert tuple(int(v) for v in z["size"]) == (128, 128, 64)
+    assert int(z["levels"]) == 4 and int(z["base_channels"]) == 16
+    assert int(z["samples"]) == 2000 and float(z["target_scale"]) > 0
+    assert np.isfinite(z["params"]).all()
+
+
+def test_warm_start_beats_cold_start_against_fp64(qesmc, cases):
+    from qesmc import warmstart as W
+
+    def first(e, t):
+        return CHUNK * (int(np.argmax(e <= t)) + 1) if (e <= t).any() else 10 ** 9
+
+    net = W.load(WEIGHTS)
+    gains, its = [], []
+    try:
+        with qesmc.Context(0) as ctx:
+            for s, city, ref in cases:
+                W.load_problem(ctx, city)
+                ctx.set_lambda(None)
+                cold = error_curve(ctx, ref, 3000)
+                ctx.set_lambda(None)
+                net.predict(ctx)
+                warm = error_curve(ctx, ref, 3000)
+                at500 = 500 // CHUNK - 1
+                # the reference schedule's 500 iterations: the warm field is closer to the converged one
+                assert warm[at500] < 0.6 * cold[at500], (s, warm[at500], cold[at500])
+                # never more iterations to 1e-3 m/s
+                assert first(warm, 1e-3) <= first(cold, 1e-3), s
+                # accuracy is not traded: both reach 1e-4 m/s of the FP64 field
+                assert (cold <= 1e-4).any() and (warm <= 1e-4).any(), (s, cold.min(), warm.min())
+                gains.append(cold[at500] / warm[at500])
+                its.append((first(cold, 1e-3), first(warm, 1e-3)))
+    finally:
+        net.close()
+    # over the held-out set, fewer iterations in total
+    assert sum(w for _, w in its) < sum(c for c, _ in its), its
+    print("error ratio cold/warm after 500 iterations:", [round(g, 2) for g in gains],
+          "iterations to 1e-3 m/s (cold, warm):", its)
