@@ -386,7 +386,7 @@ def main():
     ap.add_argument("--workload", choices=["config2", "large"], default="config2",
                     help="large: only the 1000x1000x150 8-direction ensemble (minutes of GPU time)")
     ap.add_argument("--large-dirs", type=int, default=8)
-    ap.add_argument("--large-tol", type=float, default=1e-3, help="stop rule of the large workload")
+    ap.add_argument("--large-tol", type=float, default=2e-4, help="stop rule of the large workload")
     ap.add_argument("--large-itermax", type=int, default=200000)
     ap.add_argument("--large-cpu-iters", type=int, default=20)
     ap.add_argument("--large-unet", action="store_true", help="U-Net lambda0 for the two basis solves")

@@ -49,6 +49,11 @@ __device__ inline float wave_max_b(float v)
 }
 
 // t = t_base[0] + t_off when t_base is given (graph replay), else t_off.
+//
+// Each XCD owns a contiguous eighth of the global tile range, and each wave of
+// the XCD a contiguous run of it, so a wave crosses at most a few instance
+// boundaries per pass: the instance lookup, its gate and the publication of
+// its max|dlambda| happen once per (wave, instance) run, not per tile.
 __global__ __launch_bounds__(256) void k_sor_batch(const BatchItem *__restrict__ items, int n, long long total,
                                                    int colour, const int *t_base, int t_off, int itermax,
                                                    float omega, float one_m_omega, float tol)
@@ -60,12 +65,16 @@ __global__ __launch_bounds__(256) void k_sor_batch(const BatchItem *__restrict__
   const int local = blockIdx.x / kXcdsB;
   const int nblk = (gridDim.x - xcd + kXcdsB - 1) / kXcdsB;
   const long long per = (total + kXcdsB - 1) / kXcdsB;
-  const long long tbeg = xcd * per;
-  const long long tend = tbeg + per < total ? tbeg + per : total;
-  int cur = -1;
-  bool open = false;
-  for (long long wt = tbeg + (long long)local * 4 + wave; wt < tend; wt += (long long)nblk * 4) {
-    // instance of this tile (tile offsets are increasing)
+  const long long xbeg = xcd * per;
+  const long long xend = xbeg + per < total ? xbeg + per : total;
+  if (xend <= xbeg) return;
+  const long long nwaves = (long long)nblk * 4;
+  const long long run = (xend - xbeg + nwaves - 1) / nwaves;
+  const long long wbeg = xbeg + ((long long)local * 4 + wave) * run;
+  const long long wend = wbeg + run < xend ? wbeg + run : xend;
+  long long wt = wbeg;
+  while (wt < wend) {
+    // instance of tile wt (tile offsets are increasing), and where it ends
     int lo = 0, hi = n - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
@@ -73,26 +82,34 @@ __global__ __launch_bounds__(256) void k_sor_batch(const BatchItem *__restrict__
       else hi = mid - 1;
     }
     const BatchItem &it = items[lo];
-    if (lo != cur) {
-      cur = lo;
-      open = wave_max_b(__uint_as_float(it.err[(long long)t * kErrSlots + lane])) > tol;
+    const long long iend = lo + 1 < n ? items[lo + 1].tile_begin : total;
+    const long long stop = iend < wend ? iend : wend;
+    const bool open = wave_max_b(__uint_as_float(it.err[(long long)t * kErrSlots + lane])) > tol;
+    if (!open) {
+      wt = stop;
+      continue;
     }
-    if (!open) continue;
-    const Dims &d = it.d;
-    const long long lt = wt - it.tile_begin;
-    const int kc = (int)(lt / it.tiles_per_plane);
-    const int pt = (int)(lt - (long long)kc * it.tiles_per_plane);
-    const int p = pt * 64 + lane;
-    const int jr = p / d.HP;
-    const int j = jr + 1;
-    const int q = p - jr * d.HP;
+    const Dims d = it.d;
+    float *__restrict__ own = it.lc[colour];
+    const float *__restrict__ oth = it.lc[colour ^ 1];
+    const float *__restrict__ Rc = it.Rc[colour];
+    const unsigned char *__restrict__ mk = it.mask[colour];
+    const unsigned *__restrict__ canon = it.canon;
+    const bool masked = it.masked != 0;
+    const int kb = it.kb, tpp = it.tiles_per_plane;
     float dl = 0.f;
-    if (j <= d.NY - 2) {
-      float *__restrict__ own = it.lc[colour];
-      const float *__restrict__ oth = it.lc[colour ^ 1];
-      const float *__restrict__ Rc = it.Rc[colour];
-      const int k0 = 1 + kc * it.kb;
-      const int k1 = k0 + it.kb < d.NZ - 1 ? k0 + it.kb : d.NZ - 1;
+    for (; wt < stop; ++wt) {
+      const long long lt = wt - it.tile_begin;
+      const int kc = (int)(lt / tpp);
+      const int pt = (int)(lt - (long long)kc * tpp);
+      const int p = pt * 64 + lane;
+      const int jr = p / d.HP;
+      const int j = jr + 1;
+      const int q = p - jr * d.HP;
+      if (j > d.NY - 2) continue;
+      const int k0 = 1 + kc * kb;
+      const int k1 = k0 + kb < d.NZ - 1 ? k0 + kb : d.NZ - 1;
+#pragma unroll 1
       for (int k = k0; k < k1; ++k) {
         const int s = (j + k + colour) & 1;
         const long long x = (long long)(k * d.NY + j) * d.HP + q;
@@ -104,15 +121,16 @@ __global__ __launch_bounds__(256) void k_sor_batch(const BatchItem *__restrict__
         const float xp = s ? oth[x + 1] : c;
         const float yp = oth[x + d.HP], ym = oth[x - d.HP];
         const float zp = oth[x + d.splane], zm = oth[x - d.splane];
+        const float r = Rc[x];
         float ec, fc, gc, hc, mc, nc;
-        if (it.masked) {
-          const unsigned m = it.mask[colour][x];
-          ec = (m & 1u) ? __uint_as_float(it.canon[0 * d.NZ + k]) : 0.f;
-          fc = (m & 2u) ? __uint_as_float(it.canon[1 * d.NZ + k]) : 0.f;
-          gc = (m & 4u) ? __uint_as_float(it.canon[2 * d.NZ + k]) : 0.f;
-          hc = (m & 8u) ? __uint_as_float(it.canon[3 * d.NZ + k]) : 0.f;
-          mc = (m & 16u) ? __uint_as_float(it.canon[4 * d.NZ + k]) : 0.f;
-          nc = (m & 32u) ? __uint_as_float(it.canon[5 * d.NZ + k]) : 0.f;
+        if (masked) {
+          const unsigned m = mk[x];
+          ec = (m & 1u) ? __uint_as_float(canon[0 * d.NZ + k]) : 0.f;
+          fc = (m & 2u) ? __uint_as_float(canon[1 * d.NZ + k]) : 0.f;
+          gc = (m & 4u) ? __uint_as_float(canon[2 * d.NZ + k]) : 0.f;
+          hc = (m & 8u) ? __uint_as_float(canon[3 * d.NZ + k]) : 0.f;
+          mc = (m & 16u) ? __uint_as_float(canon[4 * d.NZ + k]) : 0.f;
+          nc = (m & 32u) ? __uint_as_float(canon[5 * d.NZ + k]) : 0.f;
         } else {
           ec = upd ? it.cc[colour][0][x] : 0.f;
           fc = upd ? it.cc[colour][1][x] : 0.f;
@@ -123,12 +141,12 @@ __global__ __launch_bounds__(256) void k_sor_batch(const BatchItem *__restrict__
         }
         if (upd) {
           const float nv = (omega / (ec + fc + gc + hc + mc + nc))
-                               * (ec * xp + fc * xm + gc * yp + hc * ym + mc * zp + nc * zm - Rc[x])
+                               * (ec * xp + fc * xm + gc * yp + hc * ym + mc * zp + nc * zm - r)
                            + one_m_omega * o;
           own[x] = nv;
           if (!isfinite(nv)) atomicOr(it.status, ST_NONFINITE);
           const float dd = fabsf(nv - o);
-          dl = (dd > dl) ? dd : dl;
+          dl = (dd > dl) ? dd : dl;  // NaN never wins
         }
         // k = 0 mirror kept in the other colour (see the single-problem kernel)
         if (k == 1 && !(colour == 0 && t == 0)) {
@@ -137,9 +155,10 @@ __global__ __launch_bounds__(256) void k_sor_batch(const BatchItem *__restrict__
         }
       }
     }
+    // one publication per (wave, instance) run, striped over the slots
     const float wm = wave_max_b(dl);
     if (lane == 0 && wm > 0.f)
-      atomicMax(it.err + (long long)(t + 1) * kErrSlots + (wt % kErrSlots), __float_as_uint(wm));
+      atomicMax(it.err + (long long)(t + 1) * kErrSlots + (blockIdx.x * 4 + wave) % kErrSlots, __float_as_uint(wm));
   }
 }
 
