@@ -292,6 +292,12 @@ int qesmc_unet_clear_samples(qesmc_unet *net);
  * learning rate lr; mean loss over the steps.  ms: device time. */
 int qesmc_unet_train(qesmc_unet *net, int steps, float lr, int first, int last,
                      unsigned long long seed, float *mean_loss, double *ms);
+/* Training loss: mean squared error of lambda0 (in target units) plus, when
+ * weight > 0, weight x the mean over blocks of block^3 voxels of the squared
+ * block-mean error -- smooth errors, which SOR removes slowest, then cost
+ * (1 + weight) times more.  block: a power of two dividing the padded
+ * volume (multiples of 2^(levels-1)); weight 0 turns the term off (default). */
+int qesmc_unet_set_loss(qesmc_unet *net, int block, float weight);
 /* Mean loss over samples [first, last) (no update). */
 int qesmc_unet_eval(qesmc_unet *net, int first, int last, float *mean_loss);
 /* Predict lambda0 for the resident problem of ctx and install it as the

@@ -382,6 +382,49 @@ __global__ __launch_bounds__(256) void k_mse(const float *y, const float *t, flo
   if (threadIdx.x == 0) atomicAdd(loss, red[0]);
 }
 
+// Block-mean loss term.  The error e = y - t (k_mse leaves gy = 2 e / V) is
+// averaged over blocks of S^3 voxels; loss += weight * mean_b(ebar_b^2) and
+// gy += weight * 2 ebar_b / (NB * S^3) for every voxel of block b.  Smooth
+// errors (which SOR removes slowest) are penalised by (1 + weight), errors
+// that change sign within a block by about 1.  One work-group per block.
+__global__ __launch_bounds__(256) void k_block_err(const float *gy, float half_v, int dx, int dy, int S,
+                                                   float w_over_nb, float *eblk, float *loss)
+{
+  const int nbx = dx / S, nby = dy / S;
+  const int b = blockIdx.x;
+  const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+  const int n = S * S * S;
+  float acc = 0.f;
+  for (int q = threadIdx.x; q < n; q += 256) {
+    const int x = q % S, y = (q / S) % S, z = q / (S * S);
+    const long long v = ((long long)(bz * S + z) * dy + (by * S + y)) * dx + (bx * S + x);
+    acc += gy[v] * half_v;
+  }
+  __shared__ float red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const float m = red[0] / (float)n;
+    eblk[b] = m;
+    if (loss) atomicAdd(loss, w_over_nb * m * m);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_block_grad(float *gy, const float *eblk, int dx, int dy, int dz, int S,
+                                                    float coef)
+{
+  const long long V = (long long)dx * dy * dz;
+  const long long v = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (v >= V) return;
+  const int x = (int)(v % dx), y = (int)((v / dx) % dy), z = (int)(v / ((long long)dx * dy));
+  const int b = ((z / S) * (dy / S) + (y / S)) * (dx / S) + x / S;
+  gy[v] += coef * eblk[b];
+}
+
 // acc += sum of t^2
 __global__ __launch_bounds__(256) void k_sumsq(const float *t, long long n, double *acc)
 {
@@ -587,6 +630,9 @@ struct qesmc_unet {
   std::vector<float *> stgt;
   double conv_flops = 0.0;       // forward FLOPs of all convolutions
   float tscale = 0.f;            // output unit (0: derive from the samples)
+  int loss_block = 0;            // block-mean loss term: block edge (0: off)
+  float loss_weight = 0.f;       //   and its weight
+  float *eblk = nullptr;         //   block means of the error
 };
 
 namespace {
@@ -799,11 +845,30 @@ void free_all(qesmc_unet *u)
     f(l.A), f(l.CAT), f(l.P), f(l.E), f(l.D), f(l.gA), f(l.gCAT), f(l.gP), f(l.gE), f(l.gD);
   }
   f(u->p), f(u->gp), f(u->m1), f(u->m2), f(u->feat), f(u->out), f(u->gout), f(u->tgt), f(u->hostbuf_dev);
-  f(u->sc), f(u->dacc), f(u->loss);
+  f(u->sc), f(u->dacc), f(u->loss), f(u->eblk);
   for (auto &q : u->sfeat) f(q);
   for (auto &q : u->stgt) f(q);
   u->sfeat.clear();
   u->stgt.clear();
+}
+
+// Loss of the current output against target t into u->loss, gradient into
+// u->gout: mean squared error plus the optional block-mean term.
+hipError_t launch_loss(qesmc_unet *u, const float *t, bool grad)
+{
+  hipStream_t s = u->stream;
+  const long long V0 = (long long)u->dx * u->dy * u->dz;
+  hipLaunchKernelGGL(k_mse, g1(V0), dim3(256), 0, s, u->out, t, 1.f / u->tscale, u->gout, V0, u->loss);
+  if (u->loss_block > 0 && u->loss_weight > 0.f) {
+    const int S = u->loss_block;
+    const long long nb = (long long)(u->dx / S) * (u->dy / S) * (u->dz / S);
+    hipLaunchKernelGGL(k_block_err, dim3((unsigned)nb), dim3(256), 0, s, u->gout, 0.5f * (float)V0, u->dx, u->dy, S,
+                       u->loss_weight / (float)nb, u->eblk, u->loss);
+    if (grad)
+      hipLaunchKernelGGL(k_block_grad, g1(V0), dim3(256), 0, s, u->gout, u->eblk, u->dx, u->dy, u->dz, S,
+                         2.f * u->loss_weight / ((float)nb * (float)(S * S * S)));
+  }
+  return hipGetLastError();
 }
 
 // features + scales of ctx's resident problem into u->feat, u->sc
@@ -1106,7 +1171,7 @@ int qesmc_unet_train(qesmc_unet *u, int steps, float lr, int first, int last, un
   for (int it = 0; it < steps; ++it) {
     const int q = pick(rng);
     UCHK(forward(u, u->sfeat[q]));
-    hipLaunchKernelGGL(k_mse, g1(V0), dim3(256), 0, s, u->out, u->stgt[q], 1.f / u->tscale, u->gout, V0, u->loss);
+    UCHK(launch_loss(u, u->stgt[q], true));
     UCHK(hipMemsetAsync(u->gp, 0, sizeof(float) * u->nparams, s));
     UCHK(backward(u, u->sfeat[q]));
     ++u->adam_t;
@@ -1128,6 +1193,25 @@ int qesmc_unet_train(qesmc_unet *u, int steps, float lr, int first, int last, un
   return QESMC_OK;
 }
 
+int qesmc_unet_set_loss(qesmc_unet *u, int block, float weight)
+{
+  if (!u || block < 0 || !(weight >= 0.f)) return QESMC_ERR_ARG;
+  if (block > 0 && ((block & (block - 1)) || u->dx % block || u->dy % block || u->dz % block)) return QESMC_ERR_ARG;
+  UCHK(hipSetDevice(u->device));
+  if (u->eblk) {
+    UCHK(hipStreamSynchronize(u->stream));
+    UCHK(hipFree(u->eblk));
+    u->eblk = nullptr;
+  }
+  u->loss_block = weight > 0.f ? block : 0;
+  u->loss_weight = u->loss_block ? weight : 0.f;
+  if (u->loss_block) {
+    const long long nb = (long long)(u->dx / block) * (u->dy / block) * (u->dz / block);
+    UCHK(hipMalloc((void **)&u->eblk, sizeof(float) * nb));
+  }
+  return QESMC_OK;
+}
+
 int qesmc_unet_target_scale(const qesmc_unet *u, float *scale)
 {
   if (!u || !scale) return QESMC_ERR_ARG;
@@ -1141,11 +1225,10 @@ int qesmc_unet_eval(qesmc_unet *u, int first, int last, float *mean_loss)
   if (!(u->tscale > 0.f)) return QESMC_ERR_STATE;  // no output unit yet: train or configure it
   UCHK(hipSetDevice(u->device));
   hipStream_t s = u->stream;
-  const long long V0 = (long long)u->dx * u->dy * u->dz;
   UCHK(hipMemsetAsync(u->loss, 0, sizeof(float), s));
   for (int q = first; q < last; ++q) {
     UCHK(forward(u, u->sfeat[q]));
-    hipLaunchKernelGGL(k_mse, g1(V0), dim3(256), 0, s, u->out, u->stgt[q], 1.f / u->tscale, u->gout, V0, u->loss);
+    UCHK(launch_loss(u, u->stgt[q], false));
   }
   float l = 0.f;
   UCHK(hipMemcpyAsync(&l, u->loss, sizeof(float), hipMemcpyDeviceToHost, s));

@@ -461,6 +461,7 @@ _UNET_SIGS = {
                          C.POINTER(C.c_float), C.POINTER(C.c_double)],
     "qesmc_unet_eval": [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_float)],
     "qesmc_unet_predict": [C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)],
+    "qesmc_unet_set_loss": [C.c_void_p, C.c_int, C.c_float],
     "qesmc_unet_target_scale": [C.c_void_p, C.POINTER(C.c_float)],
 }
 EXPORTS = EXPORTS + tuple(_UNET_SIGS)
@@ -549,6 +550,10 @@ class UNet:
         _check(_unet_lib().qesmc_unet_train(self._h, int(steps), float(lr), int(first), int(last), int(seed),
                                             C.byref(loss), C.byref(ms)), "qesmc_unet_train")
         return loss.value, ms.value
+
+    def set_loss(self, block, weight):
+        """Add weight x mean((block-mean error)^2) over block^3 blocks to the loss."""
+        _check(_unet_lib().qesmc_unet_set_loss(self._h, int(block), float(weight)), "qesmc_unet_set_loss")
 
     def eval(self, first, last):
         loss = C.c_float()

@@ -121,6 +121,8 @@ def _main(argv=None):
     tr.add_argument("--lr", type=float, default=2e-3)
     tr.add_argument("--size", type=int, nargs=3, default=[128, 128, 64])
     tr.add_argument("--solve-iters", type=int, default=2500)
+    tr.add_argument("--smooth-block", type=int, default=8, help="block edge of the block-mean loss term")
+    tr.add_argument("--smooth-weight", type=float, default=0.0, help="weight of the block-mean loss term (0: off)")
     tr.add_argument("--out", required=True)
     cp = sub.add_parser("compare", help="cold versus warm start on held-out cities")
     cp.add_argument("--weights", required=True)
@@ -134,12 +136,15 @@ def _main(argv=None):
     if a.cmd == "train":
         ncx, ncy, ncz = a.size
         net = UNet(ncx, ncy, ncz, levels=4, base_channels=16, seed=1)
+        if a.smooth_weight > 0:
+            net.set_loss(a.smooth_block, a.smooth_weight)
         with Context(0) as ctx:
             t = build_dataset(net, ctx, range(a.samples), iters=a.solve_iters, tol=1e-9, log=log)
             log(f"dataset: {a.samples} cities in {t:.1f} s")
             loss, ms = train(net, a.epochs, lr=a.lr, log=log)
             log(f"training: {int(a.epochs * a.samples)} steps, {ms / 1e3:.1f} s on the device, last loss {loss:.4e}")
-        save(net, a.out, samples=a.samples, epochs=a.epochs, final_loss=loss, train_device_s=ms / 1e3)
+        save(net, a.out, samples=a.samples, epochs=a.epochs, final_loss=loss, train_device_s=ms / 1e3,
+             smooth_block=a.smooth_block, smooth_weight=a.smooth_weight)
         log(f"saved {a.out}")
         return 0
     net = load(a.weights)

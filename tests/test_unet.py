@@ -173,3 +173,34 @@ def test_training_reduces_loss_and_warm_start_keeps_solution(qesmc, oracle):
     err = max(np.abs(uw - u64).max(), np.abs(vw - v64).max(), np.abs(ww - w64).max())
     assert err <= 1e-4, err
     net.close()
+
+
+@pytest.mark.gpu
+def test_block_mean_loss_term_bounds_and_training(qesmc, oracle):
+    """loss(weight w) = MSE + w * mean(block-mean error^2), and the block means
+    of a field never exceed its RMS (Jensen): MSE <= loss(w) <= (1 + w) MSE.
+    Training with the term still reduces it."""
+    ncx, ncy, ncz = 32, 32, 16
+    net = qesmc.UNet(ncx, ncy, ncz, levels=3, base_channels=16, seed=7)
+    with qesmc.Context(0) as ctx:
+        for s in range(6):
+            p = small_city(oracle, qesmc, 300 + s)
+            ctx.set_geometry(p)
+            ctx.set_inflow(p.u0, p.v0, p.w0)
+            ctx.set_lambda(None)
+            ctx.run(itermax=800)
+            net.add_sample(ctx)
+    net.train(0, 2e-3, 0, 6)
+    mse = net.eval(0, 6)
+    w = 3.0
+    net.set_loss(8, w)
+    lw = net.eval(0, 6)
+    assert mse <= lw <= (1 + w) * mse * (1 + 1e-5), (mse, lw)
+    assert lw > mse * (1 + 1e-3)  # an untrained network's error is partly smooth
+    with pytest.raises(qesmc.QesmcError):
+        net.set_loss(6, 1.0)  # not a power of two
+    net.train(150, 2e-3, 0, 6, seed=3)
+    assert net.eval(0, 6) < 0.7 * lw
+    net.set_loss(0, 0.0)
+    assert net.eval(0, 6) <= lw
+    net.close()
