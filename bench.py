@@ -403,7 +403,7 @@ def main():
     ap.add_argument("--tile-v", type=int, default=0, help="split path: half-cells per lane (0: library choice)")
     ap.add_argument("--wg-per-cu", type=int, default=0, help="split path: work-groups per CU (0: library choice)")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline sampling (0: skip)")
-    ap.add_argument("--ttol-cities", type=int, default=2, help="time_to_tol leg: held-out cities (0: skip)")
+    ap.add_argument("--ttol-cities", type=int, default=8, help="time_to_tol leg: held-out cities (0: skip)")
     ap.add_argument("--batch", type=int, default=64, help="batch leg: instances (0: skip)")
     ap.add_argument("--workload", choices=["config2", "large"], default="config2",
                     help="large: only the 1000x1000x150 8-direction ensemble (minutes of GPU time)")

@@ -352,6 +352,13 @@ int run_prepare(qesmc_ctx *x, const qesmc_params *pp, RunState &st)
   st.masked = x->compressible;
   x->kplanes = p.tile_k > 0 ? p.tile_k : x->kplanes_auto;
   x->tile_v = p.tile_v > 0 ? p.tile_v : x->tile_v_auto;
+  if (x->tile_v == 0) {
+    // untuned default: two half-cells per lane where that still leaves at
+    // least 8 wave tiles per CU (small domains need the parallelism more)
+    const long long flat = (long long)(x->d.NY - 2) * x->d.HP;
+    const long long nkc = (x->d.NZ - 2 + x->kplanes - 1) / (x->kplanes < 1 ? 1 : x->kplanes);
+    x->tile_v = ((flat + 127) / 128) * nkc >= 8LL * x->n_cu ? 2 : 1;
+  }
   x->wg_per_cu = p.wg_per_cu > 0 ? p.wg_per_cu : x->wg_per_cu_auto;
 
   const long long rows = (long long)p.itermax + 1;

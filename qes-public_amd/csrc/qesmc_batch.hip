@@ -50,115 +50,147 @@ __device__ inline float wave_max_b(float v)
 
 // t = t_base[0] + t_off when t_base is given (graph replay), else t_off.
 //
-// Each XCD owns a contiguous eighth of the global tile range, and each wave of
-// the XCD a contiguous run of it, so a wave crosses at most a few instance
-// boundaries per pass: the instance lookup, its gate and the publication of
-// its max|dlambda| happen once per (wave, instance) run, not per tile.
+// Tiles are assigned as in the single-problem kernel (XCD x owns the x-th
+// eighth of the global tile range; the waves of an XCD stride through it
+// together, so concurrently active waves work on neighbouring tiles).  A
+// wave's tile indices only increase, so its instance is found by advancing
+// from the previous one; the instance's gate is read and its max|dlambda| is
+// published once per instance the wave visits, not per tile.
+template <bool ALL_MASKED>
 __global__ __launch_bounds__(256) void k_sor_batch(const BatchItem *__restrict__ items, int n, long long total,
                                                    int colour, const int *t_base, int t_off, int itermax,
                                                    float omega, float one_m_omega, float tol)
 {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  // the wave index is uniform across the wave: say so, so that everything
+  // derived from the tile index (the instance and its fields) lives in
+  // scalar registers
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int t = (t_base ? t_base[0] : 0) + t_off;
   if (t >= itermax) return;
   const int xcd = blockIdx.x % kXcdsB;
   const int local = blockIdx.x / kXcdsB;
   const int nblk = (gridDim.x - xcd + kXcdsB - 1) / kXcdsB;
   const long long per = (total + kXcdsB - 1) / kXcdsB;
-  const long long xbeg = xcd * per;
-  const long long xend = xbeg + per < total ? xbeg + per : total;
-  if (xend <= xbeg) return;
-  const long long nwaves = (long long)nblk * 4;
-  const long long run = (xend - xbeg + nwaves - 1) / nwaves;
-  const long long wbeg = xbeg + ((long long)local * 4 + wave) * run;
-  const long long wend = wbeg + run < xend ? wbeg + run : xend;
-  long long wt = wbeg;
-  while (wt < wend) {
-    // instance of tile wt (tile offsets are increasing), and where it ends
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (items[mid].tile_begin <= wt) lo = mid;
-      else hi = mid - 1;
+  const long long tbeg = xcd * per;
+  const long long tend = tbeg + per < total ? tbeg + per : total;
+  const long long step = (long long)nblk * 4;
+  const int slot = (blockIdx.x * 4 + wave) % kErrSlots;
+  // the current instance, kept in (wave-uniform) registers
+  int cur = -1;
+  long long cbeg = 0, cend = 0;
+  bool open = false, masked = true;
+  int NX = 0, NY = 0, NZ = 0, HP = 0, kb = 1, tpp = 1;
+  long long splane = 0;
+  QESMC_GLOBAL float *own = nullptr;
+  const QESMC_GLOBAL float *oth = nullptr, *Rc = nullptr;
+  const QESMC_GLOBAL unsigned char *mk = nullptr;
+  const QESMC_GLOBAL unsigned *canon = nullptr;
+  unsigned *status = nullptr;
+  float dl = 0.f;
+  for (long long wt = tbeg + (long long)local * 4 + wave; wt < tend; wt += step) {
+    if (wt >= cend) {
+      // publish the finished instance, then advance to the one holding wt
+      if (cur >= 0) {
+        const float wm = wave_max_b(dl);
+        if (lane == 0 && wm > 0.f)
+          atomicMax(items[cur].err + (long long)(t + 1) * kErrSlots + slot, __float_as_uint(wm));
+      }
+      dl = 0.f;
+      int nx;
+      if (cur < 0) {
+        int lo = 0, hi = n - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (items[mid].tile_begin <= wt) lo = mid;
+          else hi = mid - 1;
+        }
+        nx = lo;
+      } else {
+        nx = cur + 1;
+        while (nx + 1 < n && items[nx + 1].tile_begin <= wt) ++nx;
+      }
+      cur = nx;
+      const BatchItem &it = items[cur];
+      cbeg = it.tile_begin;
+      cend = cur + 1 < n ? items[cur + 1].tile_begin : total;
+      open = wave_max_b(__uint_as_float(it.err[(long long)t * kErrSlots + lane])) > tol;
+      NX = it.d.NX;
+      NY = it.d.NY;
+      NZ = it.d.NZ;
+      HP = it.d.HP;
+      splane = it.d.splane;
+      kb = it.kb;
+      tpp = it.tiles_per_plane;
+      own = as_global(it.lc[colour]);
+      oth = as_global(it.lc[colour ^ 1]);
+      Rc = as_global(it.Rc[colour]);
+      mk = as_global(it.mask[colour]);
+      canon = as_global(it.canon);
+      status = it.status;
+      masked = ALL_MASKED || it.masked != 0;
     }
-    const BatchItem &it = items[lo];
-    const long long iend = lo + 1 < n ? items[lo + 1].tile_begin : total;
-    const long long stop = iend < wend ? iend : wend;
-    const bool open = wave_max_b(__uint_as_float(it.err[(long long)t * kErrSlots + lane])) > tol;
-    if (!open) {
-      wt = stop;
-      continue;
-    }
-    const Dims d = it.d;
-    float *__restrict__ own = it.lc[colour];
-    const float *__restrict__ oth = it.lc[colour ^ 1];
-    const float *__restrict__ Rc = it.Rc[colour];
-    const unsigned char *__restrict__ mk = it.mask[colour];
-    const unsigned *__restrict__ canon = it.canon;
-    const bool masked = it.masked != 0;
-    const int kb = it.kb, tpp = it.tiles_per_plane;
-    float dl = 0.f;
-    for (; wt < stop; ++wt) {
-      const long long lt = wt - it.tile_begin;
-      const int kc = (int)(lt / tpp);
-      const int pt = (int)(lt - (long long)kc * tpp);
-      const int p = pt * 64 + lane;
-      const int jr = p / d.HP;
-      const int j = jr + 1;
-      const int q = p - jr * d.HP;
-      if (j > d.NY - 2) continue;
-      const int k0 = 1 + kc * kb;
-      const int k1 = k0 + kb < d.NZ - 1 ? k0 + kb : d.NZ - 1;
+    if (!open) continue;
+    const long long lt = wt - cbeg;
+    const int kc = (int)(lt / tpp);
+    const int pt = (int)(lt - (long long)kc * tpp);
+    const int p = pt * 64 + lane;
+    const int jr = p / HP;
+    const int j = jr + 1;
+    const int q = p - jr * HP;
+    if (j > NY - 2) continue;
+    const int k0 = 1 + kc * kb;
+    const int k1 = k0 + kb < NZ - 1 ? k0 + kb : NZ - 1;
 #pragma unroll 1
-      for (int k = k0; k < k1; ++k) {
-        const int s = (j + k + colour) & 1;
-        const long long x = (long long)(k * d.NY + j) * d.HP + q;
-        const int i = 2 * q + s;
-        const bool upd = (i >= 1 && i <= d.NX - 2);
-        const float o = own[x];
-        const float c = oth[x];
-        const float xm = s ? c : oth[x - 1];
-        const float xp = s ? oth[x + 1] : c;
-        const float yp = oth[x + d.HP], ym = oth[x - d.HP];
-        const float zp = oth[x + d.splane], zm = oth[x - d.splane];
-        const float r = Rc[x];
-        float ec, fc, gc, hc, mc, nc;
-        if (masked) {
-          const unsigned m = mk[x];
-          ec = (m & 1u) ? __uint_as_float(canon[0 * d.NZ + k]) : 0.f;
-          fc = (m & 2u) ? __uint_as_float(canon[1 * d.NZ + k]) : 0.f;
-          gc = (m & 4u) ? __uint_as_float(canon[2 * d.NZ + k]) : 0.f;
-          hc = (m & 8u) ? __uint_as_float(canon[3 * d.NZ + k]) : 0.f;
-          mc = (m & 16u) ? __uint_as_float(canon[4 * d.NZ + k]) : 0.f;
-          nc = (m & 32u) ? __uint_as_float(canon[5 * d.NZ + k]) : 0.f;
-        } else {
-          ec = upd ? it.cc[colour][0][x] : 0.f;
-          fc = upd ? it.cc[colour][1][x] : 0.f;
-          gc = upd ? it.cc[colour][2][x] : 0.f;
-          hc = upd ? it.cc[colour][3][x] : 0.f;
-          mc = upd ? it.cc[colour][4][x] : 0.f;
-          nc = upd ? it.cc[colour][5][x] : 0.f;
-        }
-        if (upd) {
-          const float nv = (omega / (ec + fc + gc + hc + mc + nc))
-                               * (ec * xp + fc * xm + gc * yp + hc * ym + mc * zp + nc * zm - r)
-                           + one_m_omega * o;
-          own[x] = nv;
-          if (!isfinite(nv)) atomicOr(it.status, ST_NONFINITE);
-          const float dd = fabsf(nv - o);
-          dl = (dd > dl) ? dd : dl;  // NaN never wins
-        }
-        // k = 0 mirror kept in the other colour (see the single-problem kernel)
-        if (k == 1 && !(colour == 0 && t == 0)) {
-          const int i0 = 2 * q + (s ^ 1);
-          if (i0 >= 1 && i0 <= d.NX - 2) own[x - d.splane] = c;
-        }
+    for (int k = k0; k < k1; ++k) {
+      const int s = (j + k + colour) & 1;
+      const long long x = (long long)(k * NY + j) * HP + q;
+      const int i = 2 * q + s;
+      const bool upd = (i >= 1 && i <= NX - 2);
+      const float o = own[x];
+      const float c = oth[x];
+      const float xm = s ? c : oth[x - 1];
+      const float xp = s ? oth[x + 1] : c;
+      const float yp = oth[x + HP], ym = oth[x - HP];
+      const float zp = oth[x + splane], zm = oth[x - splane];
+      const float r = Rc[x];
+      float ec, fc, gc, hc, mc, nc;
+      if (masked) {
+        const unsigned m = mk[x];
+        ec = (m & 1u) ? __uint_as_float(canon[0 * NZ + k]) : 0.f;
+        fc = (m & 2u) ? __uint_as_float(canon[1 * NZ + k]) : 0.f;
+        gc = (m & 4u) ? __uint_as_float(canon[2 * NZ + k]) : 0.f;
+        hc = (m & 8u) ? __uint_as_float(canon[3 * NZ + k]) : 0.f;
+        mc = (m & 16u) ? __uint_as_float(canon[4 * NZ + k]) : 0.f;
+        nc = (m & 32u) ? __uint_as_float(canon[5 * NZ + k]) : 0.f;
+      } else {
+        const float *const *cc = items[cur].cc[colour];
+        ec = upd ? as_global(cc[0])[x] : 0.f;
+        fc = upd ? as_global(cc[1])[x] : 0.f;
+        gc = upd ? as_global(cc[2])[x] : 0.f;
+        hc = upd ? as_global(cc[3])[x] : 0.f;
+        mc = upd ? as_global(cc[4])[x] : 0.f;
+        nc = upd ? as_global(cc[5])[x] : 0.f;
+      }
+      if (upd) {
+        const float nv = (omega / (ec + fc + gc + hc + mc + nc))
+                             * (ec * xp + fc * xm + gc * yp + hc * ym + mc * zp + nc * zm - r)
+                         + one_m_omega * o;
+        own[x] = nv;
+        if (!isfinite(nv)) atomicOr(status, ST_NONFINITE);
+        const float dd = fabsf(nv - o);
+        dl = (dd > dl) ? dd : dl;  // NaN never wins
+      }
+      // k = 0 mirror kept in the other colour (see the single-problem kernel)
+      if (k == 1 && !(colour == 0 && t == 0)) {
+        const int i0 = 2 * q + (s ^ 1);
+        if (i0 >= 1 && i0 <= NX - 2) own[x - splane] = c;
       }
     }
-    // one publication per (wave, instance) run, striped over the slots
+  }
+  if (cur >= 0) {
     const float wm = wave_max_b(dl);
-    if (lane == 0 && wm > 0.f)
-      atomicMax(it.err + (long long)(t + 1) * kErrSlots + (blockIdx.x * 4 + wave) % kErrSlots, __float_as_uint(wm));
+    if (lane == 0 && wm > 0.f) atomicMax(items[cur].err + (long long)(t + 1) * kErrSlots + slot, __float_as_uint(wm));
   }
 }
 
@@ -291,7 +323,17 @@ int qesmc_run_batch(qesmc_ctx **ctxs, int n, const qesmc_params *params, qesmc_r
     return fail(e);
   if ((e = qesmc_launch_set_u32(lead, (unsigned *)d_t, 0u)) != hipSuccess) return fail(e);
 
-  const int wg = p.wg_per_cu > 0 ? p.wg_per_cu : lead->wg_per_cu_auto;
+  bool all_masked = true;
+  for (int i = 0; i < n; ++i) all_masked = all_masked && st[i].masked;
+  // persistent grid: never more work-groups than can be resident at once
+  int wg = p.wg_per_cu > 0 ? p.wg_per_cu : lead->wg_per_cu_auto;
+  {
+    int fit = 0;
+    const hipError_t oe = all_masked
+        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, k_sor_batch<true>, 256, 0)
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, k_sor_batch<false>, 256, 0);
+    if (oe == hipSuccess && fit > 0 && fit < wg) wg = fit;
+  }
   long long blocks = (total + 3) / 4;
   const long long cap = (long long)lead->n_cu * (wg < 1 ? 1 : wg);
   if (blocks > cap) blocks = cap;
@@ -303,9 +345,14 @@ int qesmc_run_batch(qesmc_ctx **ctxs, int n, const qesmc_params *params, qesmc_r
   if ((e = hipEventRecord(e0, bs)) != hipSuccess) return fail(e);
   if (any_runs) {
     auto launch_iter = [&](const int *tb, int toff) {
-      for (int colour = 0; colour < 2; ++colour)
-        hipLaunchKernelGGL(k_sor_batch, dim3((unsigned)blocks), dim3(256), 0, bs, d_items, n, total, colour, tb, toff,
-                           p.itermax, p.omega, omo, p.tol);
+      for (int colour = 0; colour < 2; ++colour) {
+        if (all_masked)
+          hipLaunchKernelGGL(k_sor_batch<true>, dim3((unsigned)blocks), dim3(256), 0, bs, d_items, n, total, colour,
+                             tb, toff, p.itermax, p.omega, omo, p.tol);
+        else
+          hipLaunchKernelGGL(k_sor_batch<false>, dim3((unsigned)blocks), dim3(256), 0, bs, d_items, n, total, colour,
+                             tb, toff, p.itermax, p.omega, omo, p.tol);
+      }
     };
     if (use_graph) {
       // one captured chunk of C iterations, advancing the device counter

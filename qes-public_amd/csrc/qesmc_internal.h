@@ -10,6 +10,16 @@
 
 namespace qesmc {
 
+// Pointers read from memory (tables, argument structs) are generic; casting
+// them to the global address space lets loads and stores use global rather
+// than flat instructions, as for pointers passed directly as kernel arguments.
+#define QESMC_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ inline QESMC_GLOBAL T *as_global(T *p)
+{
+  return (QESMC_GLOBAL T *)p;
+}
+
 // Status bits raised by kernels into ctx->d_status.
 enum : unsigned {
   ST_NONFINITE = 1u,      // a relaxed lambda was NaN or Inf
@@ -76,7 +86,7 @@ struct qesmc_ctx {
   int kplanes = 8;             // k planes marched by one SOR wave tile (this run)
   int kplanes_auto = 8;        // default when qesmc_params.tile_k == 0
   int tile_v = 1;              // half-cells per lane of the split kernel (this run)
-  int tile_v_auto = 1;
+  int tile_v_auto = 0;         // 0: chosen per run from the domain size
   int wg_per_cu = 8;           // resident work-groups per CU of the split kernel
   int wg_per_cu_auto = 8;
   int n_cu = 256;              // compute units of the device

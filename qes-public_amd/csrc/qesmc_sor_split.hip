@@ -102,7 +102,8 @@ __global__ __launch_bounds__(256) void k_sor_split(Dims d, int colour, int kb, i
                                                    int itermax, unsigned *status)
 {
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  // uniform across the wave: tile indices then live in scalar registers
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (t >= itermax) return;
   {
     // gate: iteration t runs iff max(err row t) > tol (row 0 = 1.0f)
@@ -213,6 +214,33 @@ __global__ __launch_bounds__(256) void k_sor_split(Dims d, int colour, int kb, i
 
 using namespace qesmc;
 
+namespace {
+
+// Work-groups of one k_sor_split variant that fit on a CU at once (queried
+// once per variant): the persistent grid never asks for more, or the extra
+// work-groups would start only after others finish.
+int resident_per_cu(bool masked, int vq)
+{
+  static int cache[2][5] = {};
+  int &c = cache[masked ? 1 : 0][vq];
+  if (c == 0) {
+    int fit = 0;
+    hipError_t e;
+    if (masked)
+      e = vq == 4 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, k_sor_split<true, 4>, 256, 0)
+        : vq == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, k_sor_split<true, 2>, 256, 0)
+                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, k_sor_split<true, 1>, 256, 0);
+    else
+      e = vq == 4 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, k_sor_split<false, 4>, 256, 0)
+        : vq == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, k_sor_split<false, 2>, 256, 0)
+                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, k_sor_split<false, 1>, 256, 0);
+    c = (e == hipSuccess && fit > 0) ? fit : 1 << 20;
+  }
+  return c;
+}
+
+}  // namespace
+
 // Launch one red-black iteration (red pass, black pass) on the split layout.
 hipError_t qesmc_launch_split_iteration(qesmc_ctx *x, bool masked, int t, int itermax, float omega,
                                         float tol)
@@ -226,7 +254,10 @@ hipError_t qesmc_launch_split_iteration(qesmc_ctx *x, bool masked, int t, int it
   const int nkc = (d.NZ - 2 + kb - 1) / kb;
   const long long ntiles = (long long)tiles_per_plane * nkc;
   long long blocks = (ntiles + 3) / 4;
-  const long long cap = (long long)x->n_cu * (x->wg_per_cu < 1 ? 1 : x->wg_per_cu);
+  int wg = x->wg_per_cu < 1 ? 1 : x->wg_per_cu;
+  const int fit = resident_per_cu(masked, vq == 4 ? 4 : vq == 2 ? 2 : 1);
+  if (fit < wg) wg = fit;
+  const long long cap = (long long)x->n_cu * wg;
   if (blocks > cap) blocks = cap;
   if (blocks < kXcds) blocks = kXcds;  // every XCD's tile range needs a work-group
   const float omo = 1.0f - omega;

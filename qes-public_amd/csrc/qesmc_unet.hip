@@ -128,7 +128,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd(ConvFwdArgs a)
         if (m >= V) continue;
         float v = acc[t][n][i] + b;
         if (a.relu) v = v > 0.f ? v : 0.f;
-        a.y[m * a.ldy + a.ycoff + col] = f2bf(v);
+        as_global(a.y)[m * a.ldy + a.ycoff + col] = f2bf(v);
       }
     }
 }
