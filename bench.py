@@ -481,6 +481,8 @@ def main():
             results.append(ctx.run(params))
         wall = time.perf_counter() - t0
         div_ref = ctx.divergence_linf(False)
+        # velocity gradient tensor of the corrected field (QES-Turb derivative step)
+        grad_ms = min(ctx.velocity_gradient(uniform_z=False, copy=False)[1] for _ in range(3))
 
     its = results[-1]["iterations"]
     lups = city.interior_lups() * its
@@ -517,6 +519,8 @@ def main():
         "t_ref_ms": round(ms_step, 3),
         "sor_ms": round(ms_sor, 3),
         "div_linf_ref_operator": div_ref,
+        "velocity_gradient": {"ms": round(grad_ms, 4), "bytes_per_cell": 52,
+                              "achieved_gbs": round(52.0 * city.ncell / (grad_ms * 1e-3) / 1e9, 1)},
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": load_traffic(kernel), "bytes_per_lup": bpl,

@@ -224,6 +224,19 @@ int qesmc_set_city_inflow(qesmc_ctx *ctx, const qesmc_inflow *inflow);
 int qesmc_get_inputs(qesmc_ctx *ctx, float *e, float *f, float *g, float *h, float *m, float *n,
                      int *icellflag, float *u0, float *v0, float *w0);
 
+/* ---- after the solve: velocity gradient tensor (QES-Turb) --------------- */
+
+/* TURBGeneralData::derivativeVelocity (TURBGeneralData.cpp:479-552; device
+ * counterpart TURBGeneralData.cu:37-306) on the resident corrected field of
+ * the last qesmc_run / qesmc_solve, without the wall log-law corrections
+ * (:553-556).  G: NULL (result stays on the device) or 9*(nx-1)(ny-1)(nz-1)
+ * floats receiving Gxx, Gxy, Gxz, Gyx, Gyy, Gyz, Gzx, Gzy, Gzz (G_ab = d u_a /
+ * d x_b) for the fluid cells with i in [1,nx-3], j in [1,ny-3], k in [1,nz-3],
+ * 0 elsewhere.  uniform_z != 0 selects the reference's uniform-grid branch
+ * (flagUniformZGrid) with spacing dz; otherwise the stretched-grid formulas
+ * on the resident dz_array.  ms (optional): device time of the kernel. */
+int qesmc_velocity_gradient(qesmc_ctx *ctx, int uniform_z, float dz, float *G, double *ms);
+
 /* ---- ensembles on one geometry (no counterpart in the reference) ----------
  *
  * The discrete problem is linear in (u0, v0, w0), and a single-sensor inflow

@@ -60,6 +60,7 @@ def lib():
     L.oracle_profile.argtypes = [I, f32p, I, Fl, Fl, Fl, Fl, f32p, f32p]
     L.oracle_extrude_profile.argtypes = [I, I, I, f32p, f32p, f32p, f32p, f32p]
     L.oracle_set_velocity_zero.argtypes = [I, I, I, i32p, f32p, f32p, f32p]
+    L.oracle_velocity_gradient.argtypes = [I, I, I, Fl, Fl, Fl, f32p, I, i32p, f32p, f32p, f32p, f32p]
     L.oracle_num_threads.restype = I
     L.oracle_set_num_threads.argtypes = [I]
     _lib = L
@@ -251,6 +252,17 @@ def build_city(ncx, ncy, dx, dz_array, buildings=(), profile=(1, 5.0, 10.0, 0.1,
     L.oracle_extrude_profile(nx, ny, nz, up, vp, u0, v0, w0)
     L.oracle_set_velocity_zero(nx, ny, nz, flag, u0, v0, w0)
     return Problem(nx, ny, nz, dx, dx, dz_array, *coefs, flag, u0, v0, w0)
+
+
+def velocity_gradient(p: Problem, u, v, w, uniform_z=False, dz=None):
+    """TURBGeneralData::derivativeVelocity (TURBGeneralData.cpp:479-552):
+    returns a (9, ncell) array Gxx, Gxy, Gxz, Gyx, Gyy, Gyz, Gzx, Gzy, Gzz."""
+    G = np.empty((9, p.ncell), np.float32)
+    dzv = float(p.dz_array[1]) if dz is None else float(dz)
+    lib().oracle_velocity_gradient(p.nx, p.ny, p.nz, p.dx, p.dy, dzv, p.dz_array, 1 if uniform_z else 0,
+                                   p.icellflag, np.ascontiguousarray(u, np.float32),
+                                   np.ascontiguousarray(v, np.float32), np.ascontiguousarray(w, np.float32), G)
+    return G
 
 
 def set_num_threads(t: int):

@@ -653,6 +653,73 @@ void oracle_set_velocity_zero(int nx, int ny, int nz, const int *icellflag, floa
       }
 }
 
+/* ------------------------------------------------------------------------ */
+/* f4: turbulence model, velocity gradient tensor at cell centres            */
+/* ------------------------------------------------------------------------ */
+
+/* TURBGeneralData::derivativeVelocity, TURBGeneralData.cpp:479-552 (the wall
+ * log-law corrections that follow at :553-556 are not restated).  Cells:
+ * the fluid cells (icellflag not 0 or 2) with i in [1, nx-3], j in [1, ny-3],
+ * k in [1, nz-3] (the icellfluid list, TURBGeneralData.cpp:264-283); every
+ * other cell of the nine outputs is set to 0.  uniform_z selects the
+ * uniform-grid branch (:521-533, dz = domain.dz()), else the stretched one
+ * (:534-551) with z the cell-centre heights of Domain::defineVerticalGrid.
+ * Float throughout; 4.0f and 0.5f as in the source.  G holds the nine arrays
+ * in the order Gxx, Gxy, Gxz, Gyx, Gyy, Gyz, Gzx, Gzy, Gzz.
+ * Pinned by: linear velocity fields give their exact gradients on uniform
+ * and stretched grids (the stretched dudz/dvdz weights are exact for linear
+ * profiles), with distinct coefficients per component so that a swapped
+ * component or operand fails (tests/test_oracle.py). */
+void oracle_velocity_gradient(int nx, int ny, int nz, float dx, float dy, float dz, const float *dz_array,
+                              int uniform_z, const int *icellflag, const float *u, const float *v,
+                              const float *w, float *G)
+{
+  const long ncell = (long)(nx - 1) * (ny - 1) * (nz - 1);
+  float *Gxx = G, *Gxy = G + ncell, *Gxz = G + 2 * ncell, *Gyx = G + 3 * ncell, *Gyy = G + 4 * ncell;
+  float *Gyz = G + 5 * ncell, *Gzx = G + 6 * ncell, *Gzy = G + 7 * ncell, *Gzz = G + 8 * ncell;
+  float *z_face = (float *)malloc(sizeof(float) * nz), *z = (float *)malloc(sizeof(float) * (nz - 1));
+  oracle_vertical_grid(nz, dz_array, z_face, z);
+  memset(G, 0, sizeof(float) * 9 * ncell);
+#pragma omp parallel for schedule(static)
+  for (int k = 1; k < nz - 2; ++k)
+    for (int j = 1; j < ny - 2; ++j)
+      for (int i = 1; i < nx - 2; ++i) {
+        const long c = CELL(i, j, k);
+        if (icellflag[c] == 0 || icellflag[c] == 2) continue;
+        Gxx[c] = (u[FACE(i + 1, j, k)] - u[FACE(i, j, k)]) / (dx);
+        Gyx[c] = ((v[FACE(i + 1, j, k)] + v[FACE(i + 1, j + 1, k)]) - (v[FACE(i - 1, j, k)] + v[FACE(i - 1, j + 1, k)]))
+                 / (4.0f * dx);
+        Gzx[c] = ((w[FACE(i + 1, j, k)] + w[FACE(i + 1, j, k + 1)]) - (w[FACE(i - 1, j, k)] + w[FACE(i - 1, j, k + 1)]))
+                 / (4.0f * dx);
+        Gxy[c] = ((u[FACE(i, j + 1, k)] + u[FACE(i + 1, j + 1, k)]) - (u[FACE(i, j - 1, k)] + u[FACE(i + 1, j - 1, k)]))
+                 / (4.0f * dy);
+        Gyy[c] = (v[FACE(i, j + 1, k)] - v[FACE(i, j, k)]) / (dy);
+        Gzy[c] = ((w[FACE(i, j + 1, k)] + w[FACE(i, j + 1, k + 1)]) - (w[FACE(i, j - 1, k)] + w[FACE(i, j - 1, k + 1)]))
+                 / (4.0f * dy);
+        if (uniform_z) {
+          Gxz[c] = ((u[FACE(i, j, k + 1)] + u[FACE(i + 1, j, k + 1)]) - (u[FACE(i, j, k - 1)] + u[FACE(i + 1, j, k - 1)]))
+                   / (4.0f * dz);
+          Gyz[c] = ((v[FACE(i, j, k + 1)] + v[FACE(i, j + 1, k + 1)]) - (v[FACE(i, j, k - 1)] + v[FACE(i, j + 1, k - 1)]))
+                   / (4.0f * dz);
+          Gzz[c] = (w[FACE(i, j, k + 1)] - w[FACE(i, j, k)]) / (dz);
+        } else {
+          Gxz[c] = (0.5f * (z[k] - z[k - 1]) / (z[k + 1] - z[k])
+                      * ((u[FACE(i, j, k + 1)] + u[FACE(i + 1, j, k + 1)]) - (u[FACE(i, j, k)] + u[FACE(i + 1, j, k)]))
+                    + 0.5f * (z[k + 1] - z[k]) / (z[k] - z[k - 1])
+                        * ((u[FACE(i, j, k)] + u[FACE(i + 1, j, k)]) - (u[FACE(i, j, k - 1)] + u[FACE(i + 1, j, k - 1)])))
+                   / (z[k + 1] - z[k - 1]);
+          Gyz[c] = (0.5f * (z[k] - z[k - 1]) / (z[k + 1] - z[k])
+                      * ((v[FACE(i, j, k + 1)] + v[FACE(i, j + 1, k + 1)]) - (v[FACE(i, j, k)] + v[FACE(i, j + 1, k)]))
+                    + 0.5f * (z[k + 1] - z[k]) / (z[k] - z[k - 1])
+                        * ((v[FACE(i, j, k)] + v[FACE(i, j + 1, k)]) - (v[FACE(i, j, k - 1)] + v[FACE(i, j + 1, k - 1)])))
+                   / (z[k + 1] - z[k - 1]);
+          Gzz[c] = (w[FACE(i, j, k + 1)] - w[FACE(i, j, k)]) / (dz_array[k]);
+        }
+      }
+  free(z_face);
+  free(z);
+}
+
 int oracle_num_threads(void)
 {
 #ifdef _OPENMP

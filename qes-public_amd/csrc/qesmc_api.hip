@@ -75,6 +75,8 @@ void free_geometry(qesmc_ctx *x)
   dfree(x->d_w);
   dfree(x->d_R);
   dfree(x->d_lam);
+  dfree(x->d_grad);
+  x->have_velocity = false;
   for (int c = 0; c < 2; ++c) {
     dfree(x->d_lc[c]);
     dfree(x->d_Rc[c]);
@@ -172,6 +174,7 @@ int geometry_reserve(qesmc_ctx *x, int nx, int ny, int nz)
 int geometry_commit(qesmc_ctx *x)
 {
   x->have_geom = false;
+  x->have_velocity = false;
   int rc = encode_stencil(x);
   if (rc != QESMC_OK) return rc;
   x->have_geom = true;
@@ -392,6 +395,7 @@ int run_finish(qesmc_ctx *x, const RunState &st, int launched, qesmc_result *res
   HIPCHK(qesmc_launch_finish(x, st.first_runs, p.alpha1, p.alpha2));
   HIPCHK(hipEventRecord(x->ev[3], s));
   HIPCHK(hipStreamSynchronize(s));
+  x->have_velocity = true;
 
   std::vector<unsigned> herr(((size_t)launched + 1) * kErrSlots);
   HIPCHK(hipMemcpy(herr.data(), x->d_err, sizeof(unsigned) * herr.size(), hipMemcpyDeviceToHost));

@@ -57,6 +57,7 @@ struct qesmc_ctx {
   long long ncell = 0, nface = 0, nsplit = 0;
   std::vector<float> dz_host;
   bool have_geom = false, have_inflow = false;
+  bool have_velocity = false;  // d_u, d_v, d_w hold a corrected field of the resident geometry
   bool geom_checked = false;   // stencil encoding built for the current geometry
   bool compressible = false;
 
@@ -68,6 +69,7 @@ struct qesmc_ctx {
   float *d_u = nullptr, *d_v = nullptr, *d_w = nullptr;
   float *d_R = nullptr;
   float *d_lam = nullptr;
+  float *d_grad = nullptr;     // velocity gradient tensor, 9 cell arrays (allocated on first use)
 
   // colour-split arrays (index = colour)
   float *d_lc[2] = {};

@@ -337,6 +337,17 @@ class Context:
                "qesmc_get_inputs")
         return City(s.nx, s.ny, s.nz, s.dx, s.dy, s.dz_array, *co, flag, u0, v0, w0)
 
+    def velocity_gradient(self, uniform_z=False, dz=None, copy=True):
+        """Velocity gradient tensor of the resident corrected field
+        (qesmc_velocity_gradient): ((9, ncell) array or None, kernel ms)."""
+        dzv = float(self.city.dz_array[1]) if dz is None else float(dz)
+        G = np.empty((9, self.city.ncell), np.float32) if copy else None
+        ms = C.c_double()
+        _check(_city_lib().qesmc_velocity_gradient(self._h, 1 if uniform_z else 0, dzv,
+                                                   G.ctypes.data_as(_fp) if copy else None, C.byref(ms)),
+               "qesmc_velocity_gradient")
+        return G, ms.value
+
     def solve(self, city: City, lam=None, params: Params | None = None, **kw):
         """One-call drop-in (qesmc_solve): returns (u, v, w, lambda, result)."""
         p = params if params is not None else default_params(**kw)
@@ -388,7 +399,7 @@ def autotune(ctx: "Context", iterations=40, params: Params | None = None, **kw) 
 
 
 EXPORTS = EXPORTS + ("qesmc_set_city", "qesmc_set_city_inflow", "qesmc_get_inputs", "qesmc_lambda_combine",
-                     "qesmc_velocity_linf_diff")
+                     "qesmc_velocity_linf_diff", "qesmc_velocity_gradient")
 
 
 class CityShape:
@@ -418,6 +429,8 @@ def _city_lib():
         L.qesmc_lambda_combine.restype = C.c_int
         L.qesmc_velocity_linf_diff.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]
         L.qesmc_velocity_linf_diff.restype = C.c_int
+        L.qesmc_velocity_gradient.argtypes = [C.c_void_p, C.c_int, C.c_float, _fp, C.POINTER(C.c_double)]
+        L.qesmc_velocity_gradient.restype = C.c_int
         L._city_sigs_set = True
     return L
 

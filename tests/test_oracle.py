@@ -410,3 +410,46 @@ def test_set_velocity_zero_only_inside_solids(oracle):
     assert np.all(U[1:4, 4:9, 4] == np.float32(-np.cos(0.0) * 3.0) * -1)
     # ground: w0 at k = 1 zeroed for interior i, j (the k = 0 ghost is solid)
     assert np.all(W[1, 1:ny - 1, 1:nx - 1] == 0)
+
+
+def _linear_faces(O, nx, ny, nz, dx, dy, dz_array, A):
+    """Face values of the linear field (u, v, w) = A @ (x, y, z) at the
+    staggered positions: u at west faces, v at south faces, w at bottom faces
+    (z of u, v: cell centres; z of w: cell faces)."""
+    z_face, z = O.vertical_grid(dz_array)
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    zc = np.append(z, z[-1] + dz_array[-1]).astype(np.float64)[k]
+    zf = z_face.astype(np.float64)[k]
+    xs = {"u": (i * dx, (j + 0.5) * dy, zc), "v": ((i + 0.5) * dx, j * dy, zc), "w": ((i + 0.5) * dx, (j + 0.5) * dy, zf)}
+    out = []
+    for row, name in zip(A, "uvw"):
+        X, Y, Z = xs[name]
+        out.append((row[0] * X + row[1] * Y + row[2] * Z).astype(np.float32).ravel())
+    return out
+
+
+@pytest.mark.parametrize("stretched,dy", [(False, 2.0), (True, 2.0), (True, 3.25)])
+def test_velocity_gradient_exact_for_linear_fields(oracle, stretched, dy):
+    """TURBGeneralData.cpp:479-552 restated: every component of a linear
+    field's gradient is recovered (distinct coefficients, so a swapped
+    component or operand fails), only fluid cells in the icellfluid range are
+    written."""
+    O = oracle
+    ncx, ncy, ncz, dx = 14, 11, 9, 2.0
+    dz = O.dz_stretched(np.geomspace(1.0, 3.0, ncz).astype(np.float32)) if stretched else O.dz_uniform(ncz, 1.5)
+    p = O.build_city(ncx, ncy, dx, dz, [(8.0, 6.0, 6.0, 4.0, 5.0, 0.0, 0.0)], (1, 5.0, 10.0, 0.1, 270.0))
+    p = O.Problem(p.nx, p.ny, p.nz, dx, dy, p.dz_array, *p.coeffs, p.icellflag, p.u0, p.v0, p.w0)
+    A = np.array([[0.7, -1.3, 0.45], [2.1, 0.35, -0.8], [-0.55, 1.6, 0.25]])
+    u, v, w = _linear_faces(O, p.nx, p.ny, p.nz, p.dx, p.dy, p.dz_array, A)
+    G = O.velocity_gradient(p, u, v, w, uniform_z=not stretched, dz=1.5)
+    NX, NY, NZ = p.nx - 1, p.ny - 1, p.nz - 1
+    flag = p.icellflag.reshape(NZ, NY, NX)
+    sel = np.zeros((NZ, NY, NX), bool)
+    sel[1:p.nz - 2, 1:p.ny - 2, 1:p.nx - 2] = True
+    sel &= (flag != 0) & (flag != 2)
+    assert sel.sum() > 100 and (~sel & (flag == 0)).any()
+    expect = A.ravel()  # Gxx Gxy Gxz Gyx Gyy Gyz Gzx Gzy Gzz
+    for n in range(9):
+        g = G[n].reshape(NZ, NY, NX)
+        assert np.allclose(g[sel], expect[n], rtol=0, atol=2e-4), (n, np.abs(g[sel] - expect[n]).max())
+        assert np.all(g[~sel] == 0)
