@@ -355,14 +355,36 @@ int run_prepare(qesmc_ctx *x, const qesmc_params *pp, RunState &st)
   st.masked = x->compressible;
   x->kplanes = p.tile_k > 0 ? p.tile_k : x->kplanes_auto;
   x->tile_v = p.tile_v > 0 ? p.tile_v : x->tile_v_auto;
-  if (x->tile_v == 0) {
-    // untuned default: two half-cells per lane where that still leaves at
-    // least 8 wave tiles per CU (small domains need the parallelism more)
+  if (x->kplanes == 0 || x->tile_v == 0) {
+    // Untuned default: the most work per wave tile (planes x half-cells per
+    // lane) that still leaves >= 6 wave tiles per CU; small domains are
+    // latency-bound and need the parallelism more than long k-marches.
     const long long flat = (long long)(x->d.NY - 2) * x->d.HP;
-    const long long nkc = (x->d.NZ - 2 + x->kplanes - 1) / (x->kplanes < 1 ? 1 : x->kplanes);
-    x->tile_v = ((flat + 127) / 128) * nkc >= 8LL * x->n_cu ? 2 : 1;
+    auto tiles = [&](int kb, int vq) {
+      return ((flat + 64LL * vq - 1) / (64LL * vq)) * ((x->d.NZ - 2 + kb - 1) / kb);
+    };
+    static const int order[][2] = {{8, 2}, {4, 2}, {8, 1}, {2, 2}, {4, 1}, {2, 1}, {1, 2}, {1, 1}};
+    int bk = 0, bv = 0;
+    long long most = -1;
+    for (const auto &o : order) {
+      if ((x->kplanes && o[0] != x->kplanes) || (x->tile_v && o[1] != x->tile_v)) continue;
+      const long long n = tiles(o[0], o[1]);
+      if (n >= 6LL * x->n_cu) {
+        bk = o[0];
+        bv = o[1];
+        break;
+      }
+      if (n > most) {
+        most = n;
+        bk = o[0];
+        bv = o[1];
+      }
+    }
+    if (!x->kplanes) x->kplanes = bk ? bk : 8;
+    if (!x->tile_v) x->tile_v = bv ? bv : 1;
   }
-  x->wg_per_cu = p.wg_per_cu > 0 ? p.wg_per_cu : x->wg_per_cu_auto;
+  // untuned: 7 resident work-groups per CU for two half-cells per lane, 8 for one
+  x->wg_per_cu = p.wg_per_cu > 0 ? p.wg_per_cu : x->wg_per_cu_auto > 0 ? x->wg_per_cu_auto : x->tile_v >= 2 ? 7 : 8;
 
   const long long rows = (long long)p.itermax + 1;
   if (x->err_cap < rows) {

@@ -15,7 +15,7 @@
 #include <cstring>
 #include <vector>
 
-#include "qesmc_internal.h"
+#include "qesmc_sor_tile.h"
 
 hipError_t qesmc_launch_set_u32(qesmc_ctx *x, unsigned *p, unsigned v);
 
@@ -80,8 +80,8 @@ __global__ __launch_bounds__(256) void k_sor_batch(const BatchItem *__restrict__
   int cur = -1;
   long long cbeg = 0, cend = 0;
   bool open = false, masked = true;
-  int NX = 0, NY = 0, NZ = 0, HP = 0, kb = 1, tpp = 1;
-  long long splane = 0;
+  Dims d{};
+  int kb = 1, tpp = 1;
   QESMC_GLOBAL float *own = nullptr;
   const QESMC_GLOBAL float *oth = nullptr, *Rc = nullptr;
   const QESMC_GLOBAL unsigned char *mk = nullptr;
@@ -115,11 +115,7 @@ __global__ __launch_bounds__(256) void k_sor_batch(const BatchItem *__restrict__
       cbeg = it.tile_begin;
       cend = cur + 1 < n ? items[cur + 1].tile_begin : total;
       open = wave_max_b(__uint_as_float(it.err[(long long)t * kErrSlots + lane])) > tol;
-      NX = it.d.NX;
-      NY = it.d.NY;
-      NZ = it.d.NZ;
-      HP = it.d.HP;
-      splane = it.d.splane;
+      d = it.d;
       kb = it.kb;
       tpp = it.tiles_per_plane;
       own = as_global(it.lc[colour]);
@@ -135,62 +131,53 @@ __global__ __launch_bounds__(256) void k_sor_batch(const BatchItem *__restrict__
     const int kc = (int)(lt / tpp);
     const int pt = (int)(lt - (long long)kc * tpp);
     const int p = pt * 64 + lane;
-    const int jr = p / HP;
+    const int jr = p / d.HP;
     const int j = jr + 1;
-    const int q = p - jr * HP;
-    if (j > NY - 2) continue;
+    const int q = p - jr * d.HP;
+    if (j > d.NY - 2) continue;
     const int k0 = 1 + kc * kb;
-    const int k1 = k0 + kb < NZ - 1 ? k0 + kb : NZ - 1;
-#pragma unroll 1
-    for (int k = k0; k < k1; ++k) {
-      const int s = (j + k + colour) & 1;
-      const long long x = (long long)(k * NY + j) * HP + q;
-      const int i = 2 * q + s;
-      const bool upd = (i >= 1 && i <= NX - 2);
-      const float o = own[x];
-      const float c = oth[x];
-      const float xm = s ? c : oth[x - 1];
-      const float xp = s ? oth[x + 1] : c;
-      const float yp = oth[x + HP], ym = oth[x - HP];
-      const float zp = oth[x + splane], zm = oth[x - splane];
-      const float r = Rc[x];
-      float ec, fc, gc, hc, mc, nc;
-      if (masked) {
-        const unsigned m = mk[x];
-        ec = (m & 1u) ? __uint_as_float(canon[0 * NZ + k]) : 0.f;
-        fc = (m & 2u) ? __uint_as_float(canon[1 * NZ + k]) : 0.f;
-        gc = (m & 4u) ? __uint_as_float(canon[2 * NZ + k]) : 0.f;
-        hc = (m & 8u) ? __uint_as_float(canon[3 * NZ + k]) : 0.f;
-        mc = (m & 16u) ? __uint_as_float(canon[4 * NZ + k]) : 0.f;
-        nc = (m & 32u) ? __uint_as_float(canon[5 * NZ + k]) : 0.f;
-      } else {
-        const float *const *cc = items[cur].cc[colour];
-        ec = upd ? as_global(cc[0])[x] : 0.f;
-        fc = upd ? as_global(cc[1])[x] : 0.f;
-        gc = upd ? as_global(cc[2])[x] : 0.f;
-        hc = upd ? as_global(cc[3])[x] : 0.f;
-        mc = upd ? as_global(cc[4])[x] : 0.f;
-        nc = upd ? as_global(cc[5])[x] : 0.f;
-      }
-      if (upd) {
-        const float nv = (omega / (ec + fc + gc + hc + mc + nc))
-                             * (ec * xp + fc * xm + gc * yp + hc * ym + mc * zp + nc * zm - r)
-                         + one_m_omega * o;
-        own[x] = nv;
-        if (!isfinite(nv)) atomicOr(status, ST_NONFINITE);
-        const float dd = fabsf(nv - o);
-        dl = (dd > dl) ? dd : dl;  // NaN never wins
-      }
-      // k = 0 mirror kept in the other colour (see the single-problem kernel)
-      if (k == 1 && !(colour == 0 && t == 0)) {
-        const int i0 = 2 * q + (s ^ 1);
-        if (i0 >= 1 && i0 <= NX - 2) own[x - splane] = c;
-      }
+    const int k1 = k0 + kb < d.NZ - 1 ? k0 + kb : d.NZ - 1;
+    // generic pointers for the call; inlined, the compiler still sees that
+    // they point to global memory
+    float *o_ = (float *)own;
+    const float *oth_ = (const float *)oth, *Rc_ = (const float *)Rc;
+    const unsigned char *mk_ = (const unsigned char *)mk;
+    const unsigned *canon_ = (const unsigned *)canon;
+    if (masked) {
+      dl = sor_tile<true, 1>(d, colour, t, j, q, k0, k1, o_, oth_, Rc_, mk_, canon_, nullptr, nullptr, nullptr,
+                             nullptr, nullptr, nullptr, omega, one_m_omega, status, dl);
+    } else if (!ALL_MASKED) {
+      const float *const *cc = items[cur].cc[colour];
+      dl = sor_tile<false, 1>(d, colour, t, j, q, k0, k1, o_, oth_, Rc_, mk_, canon_,
+                              (const float *)as_global(cc[0]), (const float *)as_global(cc[1]),
+                              (const float *)as_global(cc[2]), (const float *)as_global(cc[3]),
+                              (const float *)as_global(cc[4]), (const float *)as_global(cc[5]), omega, one_m_omega,
+                              status, dl);
     }
   }
-  if (cur >= 0) {
-    const float wm = wave_max_b(dl);
-    if (lane == 0 && wm > 0.f) atomicMax(items[cur].err + (long long)(t + 1) * kErrSlots + slot, __float_as_uint(wm));
+  // last instance of each wave: combine the work-group's four waves first, so
+  // that each instance gets one atomic per work-group (device-scope atomics on
+  // a few slots serialise at the memory side)
+  __shared__ int wcur[4];
+  __shared__ float wmx[4];
+  const float wm = cur >= 0 ? wave_max_b(dl) : 0.f;
+  if (lane == 0) {
+    wcur[wave] = cur;
+    wmx[wave] = wm;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int a = 0; a < 4; ++a) {
+      if (wcur[a] < 0) continue;
+      bool seen = false;
+      for (int b = 0; b < a; ++b) seen = seen || wcur[b] == wcur[a];
+      if (seen) continue;
+      float m = wmx[a];
+      for (int b = a + 1; b < 4; ++b)
+        if (wcur[b] == wcur[a]) m = fmaxf(m, wmx[b]);
+      if (m > 0.f)
+        atomicMax(items[wcur[a]].err + (long long)(t + 1) * kErrSlots + blockIdx.x % kErrSlots, __float_as_uint(m));
+    }
   }
 }
 
@@ -326,7 +313,7 @@ int qesmc_run_batch(qesmc_ctx **ctxs, int n, const qesmc_params *params, qesmc_r
   bool all_masked = true;
   for (int i = 0; i < n; ++i) all_masked = all_masked && st[i].masked;
   // persistent grid: never more work-groups than can be resident at once
-  int wg = p.wg_per_cu > 0 ? p.wg_per_cu : lead->wg_per_cu_auto;
+  int wg = p.wg_per_cu > 0 ? p.wg_per_cu : lead->wg_per_cu_auto > 0 ? lead->wg_per_cu_auto : 8;
   {
     int fit = 0;
     const hipError_t oe = all_masked
@@ -409,8 +396,8 @@ int qesmc_autotune(qesmc_ctx *x, const qesmc_params *base, int iterations, qesmc
   float *saved = nullptr;
   BCHK(hipMalloc((void **)&saved, sizeof(float) * (size_t)x->ncell));
   BCHK(hipMemcpyAsync(saved, x->d_lam, sizeof(float) * (size_t)x->ncell, hipMemcpyDeviceToDevice, x->stream));
-  static const int cand[][3] = {{8, 1, 8}, {4, 1, 8}, {16, 1, 8}, {8, 1, 4}, {8, 1, 16},
-                                {4, 4, 8}, {8, 4, 8}, {8, 2, 8}, {2, 1, 8}, {32, 1, 8}};
+  static const int cand[][3] = {{8, 2, 8}, {8, 1, 8}, {4, 2, 8}, {4, 2, 4}, {4, 1, 8}, {2, 2, 8},
+                                {2, 1, 8}, {1, 2, 8}, {16, 1, 8}, {8, 2, 4}, {8, 1, 16}, {8, 4, 8}};
   double best_ms = 1e300;
   int bi = 0, rc = QESMC_OK;
   for (int c = 0; c < (int)(sizeof(cand) / sizeof(cand[0])); ++c) {

@@ -86,11 +86,11 @@ struct qesmc_ctx {
   double *d_norm = nullptr;
   unsigned *h_poll = nullptr;  // pinned host words: 2 polling rows + status
   int kplanes = 8;             // k planes marched by one SOR wave tile (this run)
-  int kplanes_auto = 8;        // default when qesmc_params.tile_k == 0
+  int kplanes_auto = 0;        // default when qesmc_params.tile_k == 0 (0: chosen per run)
   int tile_v = 1;              // half-cells per lane of the split kernel (this run)
   int tile_v_auto = 0;         // 0: chosen per run from the domain size
   int wg_per_cu = 8;           // resident work-groups per CU of the split kernel
-  int wg_per_cu_auto = 8;
+  int wg_per_cu_auto = 0;      // 0: chosen per run from tile_v
   int n_cu = 256;              // compute units of the device
 };
 

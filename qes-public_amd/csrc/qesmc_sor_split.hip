@@ -17,64 +17,13 @@
 //
 // Arithmetic is the reference's operand for operand
 // (src/winds/Solver_CPU_RB.cpp:109-151, compiled without contraction).
-#include "qesmc_internal.h"
+#include "qesmc_sor_tile.h"
 
 namespace qesmc {
 
 namespace {
 
 constexpr int kXcds = 8;  // MI355X: 8 XCDs, work-groups dispatched round-robin
-
-template <int VQ>
-struct Vec;
-template <>
-struct Vec<1> {
-  __device__ static inline void load(const float *p, float *v) { v[0] = p[0]; }
-  __device__ static inline void store(float *p, const float *v) { p[0] = v[0]; }
-  __device__ static inline void loadb(const unsigned char *p, unsigned *v) { v[0] = p[0]; }
-};
-template <>
-struct Vec<2> {
-  __device__ static inline void load(const float *p, float *v)
-  {
-    const float2 t = *reinterpret_cast<const float2 *>(p);
-    v[0] = t.x;
-    v[1] = t.y;
-  }
-  __device__ static inline void store(float *p, const float *v)
-  {
-    *reinterpret_cast<float2 *>(p) = make_float2(v[0], v[1]);
-  }
-  __device__ static inline void loadb(const unsigned char *p, unsigned *v)
-  {
-    const unsigned short t = *reinterpret_cast<const unsigned short *>(p);
-    v[0] = t & 0xffu;
-    v[1] = t >> 8;
-  }
-};
-template <>
-struct Vec<4> {
-  __device__ static inline void load(const float *p, float *v)
-  {
-    const float4 t = *reinterpret_cast<const float4 *>(p);
-    v[0] = t.x;
-    v[1] = t.y;
-    v[2] = t.z;
-    v[3] = t.w;
-  }
-  __device__ static inline void store(float *p, const float *v)
-  {
-    *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]);
-  }
-  __device__ static inline void loadb(const unsigned char *p, unsigned *v)
-  {
-    const unsigned t = *reinterpret_cast<const unsigned *>(p);
-    v[0] = t & 0xffu;
-    v[1] = (t >> 8) & 0xffu;
-    v[2] = (t >> 16) & 0xffu;
-    v[3] = t >> 24;
-  }
-};
 
 __device__ inline float wave_max(float v)
 {
@@ -128,76 +77,8 @@ __global__ __launch_bounds__(256) void k_sor_split(Dims d, int colour, int kb, i
     if (j > d.NY - 2) continue;
     const int k0 = 1 + kc * kb;
     const int k1 = k0 + kb < d.NZ - 1 ? k0 + kb : d.NZ - 1;
-#pragma unroll 1
-    for (int k = k0; k < k1; ++k) {
-      const int s = (j + k + colour) & 1;
-      const long long x = (long long)(k * d.NY + j) * d.HP + q;
-      float o[VQ], r[VQ], c[VQ], yp[VQ], ym[VQ], zp[VQ], zm[VQ];
-      Vec<VQ>::load(own + x, o);
-      Vec<VQ>::load(Rc + x, r);
-      Vec<VQ>::load(oth + x, c);
-      Vec<VQ>::load(oth + x + d.HP, yp);
-      Vec<VQ>::load(oth + x - d.HP, ym);
-      Vec<VQ>::load(oth + x + d.splane, zp);
-      Vec<VQ>::load(oth + x - d.splane, zm);
-      const float left = oth[x - 1], right = oth[x + VQ];
-      unsigned mk[VQ];
-      if (MASKED) Vec<VQ>::loadb(mask + x, mk);
-      float nvv[VQ];
-      bool upd[VQ];
-#pragma unroll
-      for (int v = 0; v < VQ; ++v) {
-        const int i = 2 * (q + v) + s;
-        upd[v] = (i >= 1 && i <= d.NX - 2);
-        const float xm = s ? c[v] : (v > 0 ? c[v - 1] : left);
-        const float xp = s ? (v + 1 < VQ ? c[v + 1] : right) : c[v];
-        float ec, fc, gc, hc, mc, nc;
-        if (MASKED) {
-          const unsigned m = mk[v];
-          ec = (m & 1u) ? __uint_as_float(canon[0 * d.NZ + k]) : 0.f;
-          fc = (m & 2u) ? __uint_as_float(canon[1 * d.NZ + k]) : 0.f;
-          gc = (m & 4u) ? __uint_as_float(canon[2 * d.NZ + k]) : 0.f;
-          hc = (m & 8u) ? __uint_as_float(canon[3 * d.NZ + k]) : 0.f;
-          mc = (m & 16u) ? __uint_as_float(canon[4 * d.NZ + k]) : 0.f;
-          nc = (m & 32u) ? __uint_as_float(canon[5 * d.NZ + k]) : 0.f;
-        } else {
-          ec = upd[v] ? ce[x + v] : 0.f;
-          fc = upd[v] ? cf[x + v] : 0.f;
-          gc = upd[v] ? cg[x + v] : 0.f;
-          hc = upd[v] ? ch[x + v] : 0.f;
-          mc = upd[v] ? cm[x + v] : 0.f;
-          nc = upd[v] ? cn[x + v] : 0.f;
-        }
-        const float nv = (omega / (ec + fc + gc + hc + mc + nc))
-                             * (ec * xp + fc * xm + gc * yp[v] + hc * ym[v] + mc * zp[v] + nc * zm[v] - r[v])
-                         + one_m_omega * o[v];
-        nvv[v] = nv;
-        if (upd[v]) {
-          if (!isfinite(nv)) atomicOr(status, ST_NONFINITE);
-          const float dd = fabsf(nv - o[v]);
-          dl = (dd > dl) ? dd : dl;  // NaN never wins, as in the reference's max test
-          o[v] = nv;
-        }
-      }
-      // positions that are not updated are written back unchanged: every
-      // half-cell of this row segment belongs to this lane in this pass
-      Vec<VQ>::store(own + x, o);
-      // k = 0 mirror (Solver_CPU_RB.cpp:154-162), kept in the other colour:
-      // own plane 0 at this half-index is ghost (i0,j,0), whose only reader is
-      // cell (i0,j,1) of the other colour, read in the *next* pass.  Its value
-      // must be lambda(i0,j,1) at the end of the previous iteration, which
-      // is oth plane 1 right now (that colour is not relaxed until the next
-      // pass).  In the first red pass the ghosts keep their initial values,
-      // as the reference mirrors only after an iteration.
-      if (k == 1 && !(colour == 0 && t == 0)) {
-#pragma unroll
-        for (int v = 0; v < VQ; ++v) {
-          const int i0 = 2 * (q + v) + (s ^ 1);
-          if (i0 >= 1 && i0 <= d.NX - 2) own[x - d.splane + v] = c[v];
-        }
-      }
-      (void)nvv;
-    }
+    dl = sor_tile<MASKED, VQ>(d, colour, t, j, q, k0, k1, own, oth, Rc, mask, canon, ce, cf, cg, ch, cm, cn, omega,
+                              one_m_omega, status, dl);
   }
   // block max -> one striped slot
   __shared__ float wmax[4];
