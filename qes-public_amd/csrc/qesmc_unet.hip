@@ -69,6 +69,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd(ConvFwdArgs a)
   const long long mbase = ((long long)blockIdx.x * 4 + wave) * (16 * MT);
   if (mbase >= V) return;
   const int r = lane & 15, oct = lane >> 4;
+  const int nb = blockIdx.y * NT;  // first output tile of this launch row
   int vx[MT], vy[MT], vz[MT];
   bool vin[MT];
 #pragma unroll
@@ -106,7 +107,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd(ConvFwdArgs a)
     }
 #pragma unroll
     for (int n = 0; n < NT; ++n) {
-      const bf16x8 bv = as_bf16x8(*reinterpret_cast<const uint4 *>(a.w + (long long)(16 * n + r) * a.kp + koct));
+      const bf16x8 bv = as_bf16x8(*reinterpret_cast<const uint4 *>(a.w + (long long)(16 * (nb + n) + r) * a.kp + koct));
 #pragma unroll
       for (int t = 0; t < MT; ++t) acc[t][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[t], bv, acc[t][n], 0, 0, 0);
     }
@@ -120,7 +121,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd(ConvFwdArgs a)
   for (int t = 0; t < MT; ++t)
 #pragma unroll
     for (int n = 0; n < NT; ++n) {
-      const int col = 16 * n + r;
+      const int col = 16 * (nb + n) + r;
       const float b = a.bias ? a.bias[col] : 0.f;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
@@ -129,6 +130,114 @@ __global__ __launch_bounds__(256) void k_conv_fwd(ConvFwdArgs a)
         float v = acc[t][n][i] + b;
         if (a.relu) v = v > 0.f ? v : 0.f;
         as_global(a.y)[m * a.ldy + a.ycoff + col] = f2bf(v);
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 3x3x3 convolution, forward, LDS-staged (used when cin is 8 or a multiple
+// of 16).  A work-group owns a brick of 8 x 8 x MT output voxels (4*MT
+// M-tiles of 16 voxels, MT per wave, x fastest) and all cout.  For each
+// chunk of CH input channels it stages the brick plus a one-voxel halo,
+// (10 x 10 x (MT+2)) x CH BF16, in LDS (zero outside the volume), then
+// every MFMA takes its A fragment from LDS at a per-tap constant offset:
+// with CH = 16 one 16x16x32 MFMA covers two taps x 16 channels, with CH = 8
+// four taps x 8 channels (27 taps padded to 28 with zeros).  K order
+// (tap-major, k = tap*cin + c) is the packed weights', so each lane's B
+// octet is 8 consecutive weights.
+// ---------------------------------------------------------------------------
+template <int NT, int MT, int CH>
+__global__ __launch_bounds__(256) void k_conv_fwd_lds(ConvFwdArgs a)
+{
+  constexpr int HX = 10, HY = 10, HZ = MT + 2, HV = HX * HY * HZ;  // halo brick
+  constexpr int OPT = CH / 8;                                      // lane octets per tap
+  constexpr int TPM = 4 / OPT;                                     // taps per MFMA
+  constexpr int NG = (27 + TPM - 1) / TPM;                         // MFMAs per channel chunk
+  __shared__ __attribute__((aligned(16))) bf16_t lds[HV * CH];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = lane & 15, oct = lane >> 4;
+  const int nbx = (a.dx + 7) / 8, nby = (a.dy + 7) / 8;
+  const int bx = blockIdx.x % nbx, by = (blockIdx.x / nbx) % nby, bz = blockIdx.x / (nbx * nby);
+  const int x0 = bx * 8, y0 = by * 8, z0 = bz * MT;
+  // this lane's A rows: voxel of row r in each of the wave's MT M-tiles
+  int abase[MT];
+#pragma unroll
+  for (int t = 0; t < MT; ++t) {
+    const int v = 16 * (wave * MT + t) + r;
+    const int lx = v & 7, ly = (v >> 3) & 7, lz = v >> 6;
+    abase[t] = (lz * HY + ly) * HX + lx;  // halo origin is (-1,-1,-1)
+  }
+  f32x4 acc[MT][NT];
+#pragma unroll
+  for (int t = 0; t < MT; ++t)
+#pragma unroll
+    for (int n = 0; n < NT; ++n) acc[t][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
+  // lane octet -> (tap within the MFMA, channel octet within the chunk)
+  const int otap = oct / OPT, och = 8 * (oct % OPT);
+  for (int c0 = 0; c0 < a.cin; c0 += CH) {
+    // with one output tile the chunk's weight fragments fit in registers:
+    // load them before the staging so their latency overlaps it
+    bf16x8 bpre[NT == 1 ? NG : 1];
+    if constexpr (NT == 1) {
+#pragma unroll
+      for (int g = 0; g < NG; ++g) {
+        const int tap = g * TPM + otap;
+        bpre[g] = as_bf16x8(tap < 27 ? *reinterpret_cast<const uint4 *>(a.w + (long long)r * a.kp +
+                                                                         (long long)tap * a.cin + c0 + och)
+                                     : zero4);
+      }
+    }
+    __syncthreads();  // the previous chunk's readers are done
+    for (int e = threadIdx.x; e < HV * OPT; e += 256) {
+      const int hv = e / OPT, h = e % OPT;
+      const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
+      const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+      uint4 val = zero4;
+      if (gx >= 0 && gx < a.dx && gy >= 0 && gy < a.dy && gz >= 0 && gz < a.dz)
+        val = *reinterpret_cast<const uint4 *>(a.x + (((long long)gz * a.dy + gy) * a.dx + gx) * a.ldx + c0 + 8 * h);
+      *reinterpret_cast<uint4 *>(lds + hv * CH + 8 * h) = val;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+      const int tap = g * TPM + otap;
+      const bool live = tap < 27;
+      const int tz = tap / 9, ty = (tap / 3) % 3, tx = tap % 3;
+      const int toff = (tz * HY + ty) * HX + tx;
+      bf16x8 av[MT];
+#pragma unroll
+      for (int t = 0; t < MT; ++t)
+        av[t] = as_bf16x8(live ? *reinterpret_cast<const uint4 *>(lds + (abase[t] + toff) * CH + och) : zero4);
+      const long long kw = (long long)tap * a.cin + c0 + och;  // weight column of this lane's octet
+#pragma unroll
+      for (int n = 0; n < NT; ++n) {
+        bf16x8 bv;
+        if constexpr (NT == 1)
+          bv = bpre[g];
+        else
+          bv = as_bf16x8(live ? *reinterpret_cast<const uint4 *>(a.w + (long long)(16 * n + r) * a.kp + kw) : zero4);
+#pragma unroll
+        for (int t = 0; t < MT; ++t) acc[t][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[t], bv, acc[t][n], 0, 0, 0);
+      }
+    }
+  }
+  // epilogue: lane holds D[4*oct + i][r] of each tile
+#pragma unroll
+  for (int t = 0; t < MT; ++t)
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+      const int col = 16 * n + r;
+      const float b = a.bias ? a.bias[col] : 0.f;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int v = 16 * (wave * MT + t) + 4 * oct + i;
+        const int gx = x0 + (v & 7), gy = y0 + ((v >> 3) & 7), gz = z0 + (v >> 6);
+        if (gx >= a.dx || gy >= a.dy || gz >= a.dz) continue;
+        float val = acc[t][n][i] + b;
+        if (a.relu) val = val > 0.f ? val : 0.f;
+        const long long m = ((long long)gz * a.dy + gy) * a.dx + gx;
+        as_global(a.y)[m * a.ldy + a.ycoff + col] = f2bf(val);
       }
     }
 }
@@ -663,10 +772,51 @@ hipError_t launch_conv_fwd(hipStream_t s, const ConvFwdArgs &a, int cout)
 {
   const long long V = (long long)a.dx * a.dy * a.dz;
   const int NT = cout / 16;
-  auto go = [&](auto kern, int MT) {
+  // LDS-staged kernel: cin 8 or a multiple of 16, 16-byte aligned channel
+  // rows, and enough 8x8x4 bricks to fill the GPU (the large, shallow levels;
+  // the deep levels have too few voxels for bricks and use the kernel above)
+  const long long bxy = (long long)((a.dx + 7) / 8) * ((a.dy + 7) / 8);
+  if ((a.cin == 8 || a.cin % 16 == 0) && a.ldx % 8 == 0 && cout % 16 == 0 && bxy * ((a.dz + 3) / 4) >= 1024) {
+    auto lds = [&](auto kern, int MT) {
+      const long long nb = bxy * ((a.dz + MT - 1) / MT);
+      hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(256), 0, s, a);
+      return hipGetLastError();
+    };
+    if (a.cin == 8) {
+      switch (NT) {
+      case 1: return lds(k_conv_fwd_lds<1, 4, 8>, 4);
+      case 2: return lds(k_conv_fwd_lds<2, 4, 8>, 4);
+      default: break;
+      }
+    } else {
+      switch (NT) {
+      case 1: return lds(k_conv_fwd_lds<1, 4, 16>, 4);
+      case 2: return lds(k_conv_fwd_lds<2, 4, 16>, 4);
+      case 3: return lds(k_conv_fwd_lds<3, 4, 16>, 4);
+      case 4: return lds(k_conv_fwd_lds<4, 4, 16>, 4);
+      case 6: return lds(k_conv_fwd_lds<6, 2, 16>, 2);
+      case 8: return lds(k_conv_fwd_lds<8, 2, 16>, 2);
+      case 12: return lds(k_conv_fwd_lds<12, 2, 16>, 2);
+      default: break;
+      }
+    }
+  }
+  auto go = [&](auto kern, int MT, int rows = 1) {
     const long long waves = (V + 16LL * MT - 1) / (16LL * MT);
-    hipLaunchKernelGGL(kern, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(kern, dim3((unsigned)((waves + 3) / 4), (unsigned)rows), dim3(256), 0, s, a);
   };
+  // Small volumes (the deep levels, fewer than 2048 waves with the tiles
+  // below): one output tile per wave and fewer voxel tiles, so that up to
+  // 4096 waves cover the GPU.
+  const long long want = 4096;
+  const int mt_default = NT == 1 ? 8 : NT == 2 ? 4 : NT <= 4 ? 2 : 1;
+  if ((V + 16LL * mt_default - 1) / (16LL * mt_default) < 2048) {
+    const long long w4 = (V + 63) / 64 * NT, w2 = (V + 31) / 32 * NT;
+    if (w4 >= want) go(k_conv_fwd<1, 4>, 4, NT);
+    else if (w2 >= want) go(k_conv_fwd<1, 2>, 2, NT);
+    else go(k_conv_fwd<1, 1>, 1, NT);
+    return hipGetLastError();
+  }
   switch (NT) {
   case 1: go(k_conv_fwd<1, 8>, 8); break;
   case 2: go(k_conv_fwd<2, 4>, 4); break;
