@@ -1,7 +1,7 @@
 """Benchmark of the MI355X mass-consistency solve (one GPU).
 
 A step is one complete Solver::solve of the reference schedule on the
-400 x 400 x 120 synthetic city (configuration 2): stencil encoding, lambda
+400 x 400 x 120 synthetic city: stencil encoding, lambda
 reset, divergence R, 500 red-black SOR iterations, velocity correction and
 solid zeroing, with every input already resident in HBM.  `value` is GLUPS
 (cell updates per second / 1e9) over the timed steps.
@@ -41,7 +41,7 @@ METRIC = "time-to-tol per domain (ms) + GLUPS; U-Net warm-start vs plain SOR, sa
 HBM_PEAK_GBS = 8000.0
 
 
-def config2_city():
+def city_400x400x120():
     """400 x 400 x 120 cells, dx = dy = 2 m, dz geometric from 1 m at ratio
     1.02, 200 seeded buildings (footprints U(10,40) m, heights U(10,60) m),
     log law with u* = 0.5 m/s, z0 = 0.5 m (U_ref at 10 m chosen to give that
@@ -51,7 +51,7 @@ def config2_city():
     blds = Q.random_buildings(2024, ncx, ncy, dx, 200, 10.0, 40.0, 10.0, 60.0, margin=2)
     u_ref = 0.5 / 0.4 * math.log((10.0 + 0.5) / 0.5)
     city = Q.city_build(ncx, ncy, dx, dz, blds, (1, u_ref, 10.0, 0.5, 217.0))
-    desc = ("config2: 400x400x120 cells, dx=dy=2 m, dz 1 m x1.02 geometric, %d seeded buildings, "
+    desc = ("400x400x120 cells, dx=dy=2 m, dz 1 m x1.02 geometric, %d seeded buildings, "
             "log law u*=0.5 m/s z0=0.5 m from 217 deg; R + 500 red-black SOR iterations + correction"
             % len(blds))
     return city, desc
@@ -76,9 +76,9 @@ def _oracle():
     return O
 
 
-def fp64_checker(cities):
+def fp64_reference_fields(cities):
     """FP64 converged velocity fields (oracle red-black SOR to max|dlambda| <=
-    1e-9 in double) of the given cities: the checker of the time_to_tol leg."""
+    1e-9 in double) of the given cities, which the time_to_tol leg compares against."""
     O = _oracle()
     out = {}
     for seed, city in cities.items():
@@ -127,12 +127,12 @@ def cpu_baseline(city, budget_s):
     return {"value": round(float(np.median(rates)), 4), "unit": "GLUPS", "cores": threads, "kind": "port",
             "samples_glups": [round(r, 4) for r in rates], "cpu_model": _cpu_model(),
             "serial_single_thread_glups": round(serial, 4),
-            "sample": f"config2 domain, median of {len(rates)} samples of {per_sample} SOR iterations "
+            "sample": f"400x400x120 domain, median of {len(rates)} samples of {per_sample} SOR iterations "
                       f"(oracle/qesmc_oracle.c Solver_CPU_RB restatement, OpenMP, {threads} threads); serial "
                       f"Solver_CPU restatement on 1 thread over 5 iterations"}
 
 
-WEIGHTS = os.path.join(ROOT, "qes-public_amd", "qesmc", "weights", "unet_c4.npz")
+WEIGHTS = os.path.join(ROOT, "qes-public_amd", "qesmc", "weights", "unet_128x128x64.npz")
 BF16_PEAK_TFLOPS = 2500.0
 
 
@@ -140,7 +140,7 @@ def _linf(a, b):
     return float(np.abs(a.astype(np.float64) - b).max())
 
 
-def time_to_tol_leg(seeds, checker, tol_ms=1e-4, chunk=25, cap=6000):
+def time_to_tol_leg(seeds, fp64_ref, tol_ms=1e-4, chunk=25, cap=6000):
     """Cold versus U-Net warm start on held-out 128 x 128 x 64 cities.
 
     Pass 1 (untimed) runs SOR in chunks of `chunk` iterations and reads the
@@ -148,7 +148,7 @@ def time_to_tol_leg(seeds, checker, tol_ms=1e-4, chunk=25, cap=6000):
     against the FP64 field is <= tol_ms (and the error after the reference's
     500 iterations).  Pass 2 times that many iterations from a fresh start:
     R + SOR + correction on the device (qesmc_result.ms_total), plus the
-    U-Net inference for the warm start.  Without a checker the stop rule
+    U-Net inference for the warm start.  Without FP64 fields the stop rule
     max|dlambda| <= 1e-4 is used instead."""
     from qesmc import warmstart as W
 
@@ -158,7 +158,7 @@ def time_to_tol_leg(seeds, checker, tol_ms=1e-4, chunk=25, cap=6000):
     rows = []
     with Q.Context(0) as ctx:
         for seed in seeds:
-            city = W.c4_city(seed)
+            city = W.seeded_city(seed)
             W.load_problem(ctx, city)
             row = {"seed": seed}
             for mode in ("cold", "warm"):
@@ -166,8 +166,8 @@ def time_to_tol_leg(seeds, checker, tol_ms=1e-4, chunk=25, cap=6000):
                 if mode == "warm":
                     net.predict(ctx)
                 it, it_tol, err500, err_tol = 0, None, None, None
-                if checker is not None:
-                    u64, v64, w64, _ = checker[seed]
+                if fp64_ref is not None:
+                    u64, v64, w64, _ = fp64_ref[seed]
                     while it < cap:
                         ctx.run(itermax=chunk, tol=-1.0)
                         it += chunk
@@ -182,7 +182,7 @@ def time_to_tol_leg(seeds, checker, tol_ms=1e-4, chunk=25, cap=6000):
                 # pass 2: timed from a fresh start
                 ctx.set_lambda(None)
                 unet_ms, tflops = net.predict(ctx) if mode == "warm" else (0.0, 0.0)
-                if checker is not None and it_tol is not None:
+                if fp64_ref is not None and it_tol is not None:
                     r = ctx.run(itermax=it_tol, tol=-1.0)
                 else:
                     r = ctx.run(itermax=20000, tol=1e-4)
@@ -203,7 +203,7 @@ def time_to_tol_leg(seeds, checker, tol_ms=1e-4, chunk=25, cap=6000):
         "workload": "%d held-out seeded cities, 128x128x64 cells, dx=2 m, dz 1 m x1.02, 12-30 buildings, log law, "
                     "random direction; U-Net 4 levels 16/32/64/128 trained on 2000 solver-generated cities" % len(rows),
         "criterion": ("L_inf(u,v,w) vs FP64 oracle <= %g m/s, checked every %d iterations" % (tol_ms, chunk)
-                      if checker is not None else "stop rule max|dlambda| <= 1e-4 (no FP64 checker)"),
+                      if fp64_ref is not None else "stop rule max|dlambda| <= 1e-4 (no FP64 fields)"),
         "cold": {"iterations": mean("cold", "iterations"), "ms": cold_ms, "err_at_500": mean("cold", "err_at_500")},
         "warm": {"iterations": mean("warm", "iterations"), "ms": warm_ms, "err_at_500": mean("warm", "err_at_500"),
                  "unet_ms": mean("warm", "unet_ms")},
@@ -429,8 +429,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline sampling (0: skip)")
     ap.add_argument("--ttol-cities", type=int, default=8, help="time_to_tol leg: held-out cities (0: skip)")
     ap.add_argument("--batch", type=int, default=64, help="batch leg: instances (0: skip)")
-    ap.add_argument("--workload", choices=["config2", "large"], default="config2",
-                    help="large: only the 1000x1000x150 8-direction ensemble (minutes of GPU time)")
+    ap.add_argument("--workload", choices=["city400", "city1000"], default="city400",
+                    help="city1000: only the 1000x1000x150 8-direction ensemble (minutes of GPU time)")
     ap.add_argument("--large-dirs", type=int, default=8, help="ensemble size of the large workload")
     ap.add_argument("--large-measure", default="270,180,45,135", help="directions solved (deg); 270 and 180 are the basis")
     ap.add_argument("--large-ref-iters", type=int, default=100000, help="iterations of each direction's reference")
@@ -441,7 +441,7 @@ def main():
 
     if a.gpus != 1:
         print(f"bench.py: --gpus {a.gpus} refused: this build targets one GPU; "
-              "multi-GPU sharding is described in DESIGN.md but not built", file=sys.stderr)
+              "sharding across GPUs is not built", file=sys.stderr)
         sys.exit(2)
 
     if a.copy_calibration_mib:
@@ -458,13 +458,13 @@ def main():
         print(json.dumps({"copy_calibration_bytes": n * 4}))
         return
 
-    if a.workload == "large":
+    if a.workload == "city1000":
         measured = [float(v) for v in a.large_measure.split(",")]
         out = large_leg(a.large_dirs, measured, a.large_ref_iters, a.large_cpu_iters)
-        print(json.dumps({"metric": METRIC, "workload": "large", "n_gpus": 1, "large": out}))
+        print(json.dumps({"metric": METRIC, "workload": "city1000", "n_gpus": 1, "city1000": out}))
         return
 
-    city, desc = config2_city()
+    city, desc = city_400x400x120()
     path = {"auto": Q.PATH_AUTO, "plain": Q.PATH_PLAIN, "split": Q.PATH_SPLIT}[a.path]
     params = Q.default_params(itermax=a.itermax, tol=1e-9, path=path, flags=Q.RUN_REENCODE, tile_k=a.tile_k,
                               tile_v=a.tile_v, wg_per_cu=a.wg_per_cu)
@@ -527,18 +527,18 @@ def main():
                      "avg_launch_us": round(launch_ms * 1e3, 2)},
         "cpu_baseline": None,
     }
-    checker = None
+    fp64_ref = None
     seeds = list(range(1_000_000, 1_000_000 + a.ttol_cities))
     if a.cpu_budget > 0:
         out["cpu_baseline"] = cpu_baseline(city, a.cpu_budget)
         if seeds:
             from qesmc import warmstart as W
 
-            checker = fp64_checker({s_: W.c4_city(s_) for s_ in seeds})
+            fp64_ref = fp64_reference_fields({s_: W.seeded_city(s_) for s_ in seeds})
     if seeds:
-        out["time_to_tol"] = time_to_tol_leg(seeds, checker)
-        if checker is not None:
-            out["time_to_tol"]["fp64_oracle_iterations"] = [checker[s_][3] for s_ in seeds]
+        out["time_to_tol"] = time_to_tol_leg(seeds, fp64_ref)
+        if fp64_ref is not None:
+            out["time_to_tol"]["fp64_oracle_iterations"] = [fp64_ref[s_][3] for s_ in seeds]
     if a.batch > 0:
         out["batch"] = batch_leg(a.batch)
     print(json.dumps(out))

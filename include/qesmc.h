@@ -264,7 +264,7 @@ int qesmc_velocity_linf_diff(qesmc_ctx *a, qesmc_ctx *b, double *out);
 typedef struct qesmc_unet qesmc_unet;
 
 typedef struct qesmc_unet_config {
-  int levels;          /* resolution levels (4 in the north-star network)    */
+  int levels;          /* resolution levels (4 in the shipped network)       */
   int base_channels;   /* channels at level 0, doubled per level (16)        */
   int ncx, ncy, ncz;   /* problem size in cells (fixes the activation sizes) */
   unsigned long long seed; /* weight initialisation                           */

@@ -1,7 +1,7 @@
 """ctypes wrapper over the CPU oracle (oracle/qesmc_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, always as the checker.  The product library
+cpu_baseline leg of bench.py, always as the reference they compare against.  The product library
 (libqesmc) never imports this module.
 
 Arrays follow the reference's layouts (src/qes/Domain.h:143-175): cell arrays

@@ -1,7 +1,8 @@
 /*
  * qesmc_oracle.c -- CPU restatement of QES-Winds' mass-consistency step.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/,
+ * TEST INFRASTRUCTURE ONLY.  This file is the reference results are compared
+ * against: only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.  The
  * product (libqesmc, HIP) never links, loads or calls anything here.
  *

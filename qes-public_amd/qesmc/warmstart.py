@@ -1,6 +1,6 @@
 """Learned warm start pipeline on top of the C ABI (qesmc_unet_*).
 
-  cities      seeded flat cities of one size (configuration-4 statistics)
+  cities      seeded flat 128 x 128 x 64-cell cities (12-30 buildings, log law)
   dataset     converged SOR lambda of each city, captured on the device
   training    Adam on the device, BF16 MFMA convolutions
   comparison  SOR from lambda = 0 (cold) versus SOR from the U-Net's lambda0
@@ -18,7 +18,7 @@ import numpy as np
 from . import Context, UNet, city_build, default_params, dz_geometric, random_buildings
 
 
-def c4_city(seed: int, ncx=128, ncy=128, ncz=64, dx=2.0):
+def seeded_city(seed: int, ncx=128, ncy=128, ncz=64, dx=2.0):
     """A seeded city: 12-30 non-overlapping rectangles (footprints
     U(8, 30) m, heights U(8, 40) m), dz geometric from 1 m at ratio 1.02,
     log-law inflow u* = 0.5 m/s, z0 = 0.5 m, direction uniform in 0-360 deg."""
@@ -41,7 +41,7 @@ def build_dataset(net: UNet, ctx: Context, seeds, iters=3000, tol=1e-7, log=None
     t0 = time.perf_counter()
     params = default_params(itermax=iters, tol=tol)
     for q, s in enumerate(seeds):
-        city = c4_city(s, *net_size(net))
+        city = seeded_city(s, *net_size(net))
         load_problem(ctx, city)
         ctx.set_lambda(None)
         ctx.run(params)
@@ -96,7 +96,7 @@ def compare(net: UNet, ctx: Context, seeds, tol, itermax=20000, check_every=16):
     out = []
     params = default_params(itermax=itermax, tol=tol, check_every=check_every)
     for s in seeds:
-        city = c4_city(s, *net_size(net))
+        city = seeded_city(s, *net_size(net))
         load_problem(ctx, city)
         ctx.set_lambda(None)
         cold = ctx.run(params)

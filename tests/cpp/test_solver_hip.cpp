@@ -2,7 +2,7 @@
 // a solver from SolverFactory by solveType, then per time step
 // resetLambda (or not) + solve(WGD, itermax), and checks u, v, w and lambda
 // bit for bit against the CPU oracle (test infrastructure, linked here only
-// as the checker).  Prints PASS and returns 0 on success.
+// as the reference).  Prints PASS and returns 0 on success.
 #include <cmath>
 #include <cstdio>
 #include <cstring>

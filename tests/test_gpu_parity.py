@@ -3,7 +3,7 @@
 The library keeps the reference's operand order and promotions and is built
 without FMA contraction, so the bar here is bit-exact equality of lambda, u,
 v, w, the iteration count and the last max|dlambda| with the oracle's
-restatement of Solver_CPU_RB::solve.  The north-star tolerance
+restatement of Solver_CPU_RB::solve.  The 1e-4 m/s tolerance
 (max |du,dv,dw| <= 1e-4 m/s) is asserted as well, and again against the FP64
 converged oracle where a time-to-tolerance is claimed.
 """
@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 PLAIN, SPLIT = 1, 2
 
 
-def config1(O):
+def single_cube_city(O):
     """100 x 100 x 50 cells, dx = dz = 1 m, 20 m cube centred at (50, 50),
     power law 5 (z/10)^0.15 from 270 deg."""
     return O.build_city(100, 100, 1.0, O.dz_uniform(50, 1.0), [(40.0, 40.0, 20.0, 20.0, 20.0, 0.0, 0.0)],
@@ -57,8 +57,8 @@ def check_solve(ctx, O, p, path, lam0=None, **kw):
 
 
 @pytest.mark.parametrize("path", [PLAIN, SPLIT])
-def test_config1_reference_schedule_bit_exact(gpu_ctx, oracle, path):
-    p = config1(oracle)
+def test_single_cube_100x100x50_reference_schedule_bit_exact(gpu_ctx, oracle, path):
+    p = single_cube_city(oracle)
     res = check_solve(gpu_ctx, oracle, p, path)
     assert res["iterations"] == 500 and res["path"] == path
     if path == SPLIT:
@@ -178,7 +178,7 @@ def test_resident_interface_many_directions(qesmc, oracle):
 
 
 def test_device_divergence_norms_match_oracle(gpu_ctx, oracle):
-    p = config1(oracle)
+    p = single_cube_city(oracle)
     u, v, w, lam, res = gpu_ctx.solve(p, itermax=200)
     for physical in (False, True):
         gd = gpu_ctx.divergence_linf(physical)
@@ -189,9 +189,9 @@ def test_device_divergence_norms_match_oracle(gpu_ctx, oracle):
 
 
 def test_time_to_tolerance_against_fp64_oracle(gpu_ctx, oracle):
-    """Configuration 1 run until max|dlambda| reaches fp32's floor: within
-    1e-4 m/s of the FP64 converged oracle (north-star accuracy bar)."""
-    p = config1(oracle)
+    """The 100 x 100 x 50 single-cube case run until max|dlambda| reaches
+    fp32's floor: within 1e-4 m/s of the FP64 converged oracle."""
+    p = single_cube_city(oracle)
     u64, v64, w64, lam64, it64, me64 = oracle.solve_rb_f64(p, tol=1e-9)
     assert me64 <= 1e-9
     u, v, w, lam, res = gpu_ctx.solve(p, itermax=1000, tol=1e-9)
@@ -202,7 +202,7 @@ def test_time_to_tolerance_against_fp64_oracle(gpu_ctx, oracle):
     assert gpu_ctx.divergence_linf(False) <= oracle.div_linf(p, ou, ov, ow) * (1 + 1e-12)
 
 
-def test_config2_full_size_reference_schedule(gpu_ctx, oracle, qesmc):
+def test_city_400x400x120_reference_schedule(gpu_ctx, oracle, qesmc):
     """400 x 400 x 120, dx = 2 m, geometric dz from 1 m at ratio 1.02, 200
     seeded buildings, log law (u* = 0.5 m/s, z0 = 0.5 m): 500 iterations on
     both paths, bit-exact against the oracle (run on the host cores)."""

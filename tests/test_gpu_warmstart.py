@@ -9,7 +9,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-WEIGHTS = os.path.join(ROOT, "qes-public_amd", "qesmc", "weights", "unet_c4.npz")
+WEIGHTS = os.path.join(ROOT, "qes-public_amd", "qesmc", "weights", "unet_128x128x64.npz")
 SEEDS = (1_000_000, 1_000_001, 1_000_002)
 CHUNK = 25
 
@@ -34,7 +34,7 @@ def cases(qesmc, oracle):
 
     out = []
     for s in SEEDS:
-        city = W.c4_city(s)
+        city = W.seeded_city(s)
         u, v, w, _, _, _ = oracle.solve_rb_f64(oracle.Problem(**city.arrays()), tol=1e-9)
         out.append((s, city, (u, v, w)))
     return out

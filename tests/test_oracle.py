@@ -331,7 +331,7 @@ def test_vertical_grid_closed_form(oracle):
 
 
 def test_cube_flags_and_walls_closed_form(oracle):
-    """The configuration-1 cube: x, y in [40, 60] m, 20 m high, dx = dz = 1:
+    """The single 20 m cube of the 100 x 100 x 50 case: x, y in [40, 60] m, 20 m high, dx = dz = 1:
     solid cells exactly i, j in [40, 59], k in [1, 20]; one zeroed
     coefficient per fluid face of the cube; ground walls n = 0 at k = 1."""
     O = oracle

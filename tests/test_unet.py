@@ -32,7 +32,7 @@ def test_oracle_layout_counts():
     import unet_oracle as U
 
     layers, head, n = U.layout(4, 16)
-    # the north-star network: 4 levels, 16/32/64/128 channels
+    # the shipped network: 4 levels, 16/32/64/128 channels
     assert [(l[1], l[2]) for l in layers] == [(8, 16), (16, 16), (16, 32), (32, 32), (32, 64), (64, 64), (64, 128),
                                               (128, 128), (192, 64), (64, 64), (96, 32), (32, 32), (48, 16),
                                               (16, 16)]
