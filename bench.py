@@ -339,8 +339,16 @@ def large_leg(dirs, measured, ref_iters, cpu_iters, tol_ms=1e-4):
             c.set_city(ncx, ncy, 1.0, dz, blds, (1, u_ref, 10.0, 0.5, 270.0))
         out["device_city_build_s_per_context"] = round((time.perf_counter() - t0) / 4, 3)
         lups = ref.city.interior_lups()
+        # tiling chosen by the autotuner for this domain (kept per context)
+        t0 = time.perf_counter()
+        best = None
+        for c in ctx:
+            best = Q.autotune(c, iterations=20)
+        out["autotuned_tile"] = [best.tile_k, best.tile_v, best.wg_per_cu]
+        out["autotune_s"] = round(time.perf_counter() - t0, 2)
         r = work.run(itermax=100, tol=-1.0, path=Q.PATH_PLAIN)
         plain_ms_it = r["ms_sor"] / 100
+        work.set_lambda(None)
         r = work.run(itermax=100, tol=-1.0)
         split_ms_it = r["ms_sor"] / 100
         out["ms_per_iteration"] = {"plain": round(plain_ms_it, 4), "split": round(split_ms_it, 4)}

@@ -83,7 +83,10 @@ def test_oracle_equivariances():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("levels,c0,dims", [(2, 16, (16, 12, 8)), (3, 16, (16, 16, 8)), (4, 16, (32, 16, 16)),
-                                            (3, 32, (16, 8, 16)), (4, 16, (32, 32, 32))])
+                                            (3, 32, (16, 8, 16)), (4, 16, (32, 32, 32)),
+                                            # large enough (>= 1024 bricks of 8x8x4) for the LDS-staged kernel,
+                                            # in the forward pass and in the data gradients
+                                            (2, 16, (128, 64, 64))])
 def test_device_forward_backward_match_oracle(qesmc, levels, c0, dims):
     import unet_oracle as U
 
