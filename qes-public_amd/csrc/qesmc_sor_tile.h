@@ -92,7 +92,14 @@ __device__ __forceinline__ float sor_tile(const Dims &d, int colour, int t, int 
     Vec<VQ>::load(oth + x - d.splane, zm);
     const float left = oth[x - 1], right = oth[x + VQ];
     unsigned mk[VQ];
-    if (MASKED) Vec<VQ>::loadb(mask + x, mk);
+    // the layer's six wall values, read unconditionally (uniform addresses,
+    // no per-lane branches around the loads)
+    float cv[6];
+    if (MASKED) {
+      Vec<VQ>::loadb(mask + x, mk);
+#pragma unroll
+      for (int a = 0; a < 6; ++a) cv[a] = __uint_as_float(canon[a * d.NZ + k]);
+    }
     float nvv[VQ];
     bool upd[VQ];
 #pragma unroll
@@ -104,12 +111,12 @@ __device__ __forceinline__ float sor_tile(const Dims &d, int colour, int t, int 
       float ec, fc, gc, hc, mc, nc;
       if (MASKED) {
         const unsigned m = mk[v];
-        ec = (m & 1u) ? __uint_as_float(canon[0 * d.NZ + k]) : 0.f;
-        fc = (m & 2u) ? __uint_as_float(canon[1 * d.NZ + k]) : 0.f;
-        gc = (m & 4u) ? __uint_as_float(canon[2 * d.NZ + k]) : 0.f;
-        hc = (m & 8u) ? __uint_as_float(canon[3 * d.NZ + k]) : 0.f;
-        mc = (m & 16u) ? __uint_as_float(canon[4 * d.NZ + k]) : 0.f;
-        nc = (m & 32u) ? __uint_as_float(canon[5 * d.NZ + k]) : 0.f;
+        ec = (m & 1u) ? cv[0] : 0.f;
+        fc = (m & 2u) ? cv[1] : 0.f;
+        gc = (m & 4u) ? cv[2] : 0.f;
+        hc = (m & 8u) ? cv[3] : 0.f;
+        mc = (m & 16u) ? cv[4] : 0.f;
+        nc = (m & 32u) ? cv[5] : 0.f;
       } else {
         ec = upd[v] ? ce[x + v] : 0.f;
         fc = upd[v] ? cf[x + v] : 0.f;
