@@ -59,13 +59,18 @@ def city_400x400x120():
 
 def bytes_per_lup(res):
     """Algorithmic HBM bytes per lattice update of the dominant SOR kernel
-    (one colour pass updates half the cells):
+    (a colour pass updates half the cells; a fused launch all of them):
       split, wall mask : own lambda r+w 8 + R 4 + mask 1 + other-colour lambda 4 = 17
       split, full coef : 8 + 4 + 24 + 4 = 40
+      fused, wall mask : own lambda r+w 8 + R 4 + mask 1 = 13 (each lambda is
+                         read once per iteration: the black pass takes its red
+                         neighbours from the work-group's LDS)
       plain            : coefficient and R lines of both colours are fetched by
                          each pass: 2*(24+4) + lambda r (4, neighbours shared) + w 4 = 64"""
     if res["path"] == Q.PATH_PLAIN:
         return 64.0
+    if res["path"] == Q.PATH_FUSED:
+        return 13.0
     return 17.0 if res["compressed"] else 40.0
 
 
@@ -415,6 +420,53 @@ def large_leg(dirs, measured, ref_iters, cpu_iters, tol_ms=1e-4):
     return out
 
 
+def tile_sweep_leg():
+    """Per-iteration SOR time of every tiling (tile_k, tile_v, wg_per_cu) in a
+    grid, on four domains: the 400 x 400 x 120 city, seeded 128 x 128 x 64 and
+    256 x 256 x 96 cities, and the 1000 x 1000 x 150 city (wind from 270 deg).
+    Best of 3 (2 on the large city) runs of the same iterations; the default
+    tiling is timed the same way.  Returns {domain: {"default": [tk, tv, wg,
+    us], "rows": [[us, tk, tv, wg], ...] sorted}}."""
+    from qesmc import warmstart as W
+
+    def c400(ctx):
+        city, _ = city_400x400x120()
+        ctx.set_geometry(city)
+        ctx.set_inflow(city.u0, city.v0, city.w0)
+
+    def seeded(n, nz):
+        return lambda ctx: W.load_problem(ctx, W.seeded_city(77, n, n, nz))
+
+    def c1000(ctx):
+        ncx, ncy, dz, blds, u_ref, _ = large_city(1)
+        ctx.set_city(ncx, ncy, 1.0, dz, blds, (1, u_ref, 10.0, 0.5, 270.0))
+
+    tk_all = (2, 3, 4, 5, 6, 7, 8, 10, 12, 16)
+    plan = [("400x400x120", c400, 500, 3, (1, 2, 4), tk_all, (3, 4, 5, 6, 7, 8)),
+            ("128x128x64", seeded(128, 64), 500, 3, (1, 2), (1, 2, 3, 4, 6, 8), (4, 6, 8, 10, 12)),
+            ("256x256x96", seeded(256, 96), 500, 3, (1, 2), (2, 3, 4, 6, 8, 10), (4, 5, 6, 7, 8)),
+            ("1000x1000x150", c1000, 100, 2, (1, 2), (4, 6, 8, 10, 12), (4, 5, 6, 7, 8))]
+    out = {}
+    for name, setup, its, reps, tvs, tks, wgs in plan:
+        def timed(**tile):
+            best = 1e300
+            for _ in range(reps):
+                ctx.set_lambda(None)
+                best = min(best, ctx.run(Q.default_params(itermax=its, tol=-1.0, **tile))["ms_sor"])
+            return round(best / its * 1e3, 2)
+
+        with Q.Context(0) as ctx:
+            setup(ctx)
+            d = ctx.run(Q.default_params(itermax=its, tol=-1.0))
+            rows = [[timed(tile_k=tk, tile_v=tv, wg_per_cu=wg), tk, tv, wg]
+                    for tv in tvs for tk in tks for wg in wgs]
+            out[name] = {"default": [d["tile_k"], d["tile_v"], d["wg_per_cu"], timed()],
+                         "rows": sorted(rows)}
+        print("tile sweep %s: default %s, best %s" % (name, out[name]["default"], out[name]["rows"][0]),
+              file=sys.stderr, flush=True)
+    return out
+
+
 def load_traffic(workload_key):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
@@ -430,7 +482,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--itermax", type=int, default=500)
-    ap.add_argument("--path", choices=["auto", "plain", "split"], default="auto")
+    ap.add_argument("--path", choices=["auto", "plain", "split", "fused"], default="auto")
     ap.add_argument("--tile-k", type=int, default=0, help="k planes per SOR work-group (0: library choice)")
     ap.add_argument("--tile-v", type=int, default=0, help="split path: half-cells per lane (0: library choice)")
     ap.add_argument("--wg-per-cu", type=int, default=0, help="split path: work-groups per CU (0: library choice)")
@@ -443,6 +495,8 @@ def main():
     ap.add_argument("--large-measure", default="270,180,45,135", help="directions solved (deg); 270 and 180 are the basis")
     ap.add_argument("--large-ref-iters", type=int, default=100000, help="iterations of each direction's reference")
     ap.add_argument("--large-cpu-iters", type=int, default=20)
+    ap.add_argument("--tile-sweep", action="store_true",
+                    help="only time a grid of SOR tilings on four domain sizes (one JSON line, about a minute)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run 5 plain device copies of this many MiB (HBM counter calibration under rocprofv3)")
     a = ap.parse_args()
@@ -451,6 +505,10 @@ def main():
         print(f"bench.py: --gpus {a.gpus} refused: this build targets one GPU; "
               "sharding across GPUs is not built", file=sys.stderr)
         sys.exit(2)
+
+    if a.tile_sweep:
+        print(json.dumps({"tile_sweep_us_per_iteration": tile_sweep_leg()}))
+        return
 
     if a.copy_calibration_mib:
         # A copy of known size through torch's device copy kernel: its
@@ -473,7 +531,7 @@ def main():
         return
 
     city, desc = city_400x400x120()
-    path = {"auto": Q.PATH_AUTO, "plain": Q.PATH_PLAIN, "split": Q.PATH_SPLIT}[a.path]
+    path = {"auto": Q.PATH_AUTO, "plain": Q.PATH_PLAIN, "split": Q.PATH_SPLIT, "fused": Q.PATH_FUSED}[a.path]
     params = Q.default_params(itermax=a.itermax, tol=1e-9, path=path, flags=Q.RUN_REENCODE, tile_k=a.tile_k,
                               tile_v=a.tile_v, wg_per_cu=a.wg_per_cu)
     with Q.Context(0) as ctx:
@@ -497,14 +555,17 @@ def main():
     glups = lups * a.steps / wall / 1e9
     ms_step = wall / a.steps * 1e3
     ms_sor = float(np.mean([r["ms_sor"] for r in results]))
-    launches = 2 * its
-    launch_ms = ms_sor / launches
-    bpl = bytes_per_lup(results[-1])
-    launch_bytes = bpl * city.interior_lups() / 2
-    achieved = launch_bytes / (launch_ms * 1e-3) / 1e9
     last = results[-1]
+    fused = last["path"] == Q.PATH_FUSED
+    launches = its if fused else 2 * its
+    launch_ms = ms_sor / launches
+    bpl = bytes_per_lup(last)
+    launch_bytes = bpl * city.interior_lups() / (1 if fused else 2)
+    achieved = launch_bytes / (launch_ms * 1e-3) / 1e9
     if last["path"] == Q.PATH_PLAIN:
         kernel = "k_rb_plain"
+    elif fused:
+        kernel = "k_sor_fused"
     else:
         kernel = "k_sor_split<%s, %d>" % ("true" if last["compressed"] else "false", last["tile_v"])
     out = {

@@ -71,6 +71,10 @@ typedef struct qesmc_fields {
 #define QESMC_PATH_SPLIT 2    /* colour-split (checkerboard) layout, per-cell
                                  6-bit wall mask when the coefficients are
                                  stair-step, full coefficients otherwise        */
+#define QESMC_PATH_FUSED 3    /* colour-split layout and wall mask, both colour
+                                 passes of an iteration in one launch (two
+                                 lambda buffers); stair-step coefficients only,
+                                 else QESMC_ERR_UNSUPPORTED                     */
 
 typedef struct qesmc_params {
   float omega;      /* over-relaxation, reference 1.78f (Solver.h:83)          */
@@ -95,7 +99,7 @@ typedef struct qesmc_params {
 
 typedef struct qesmc_result {
   int iterations;            /* SOR iterations executed (reference prints it, :192) */
-  int path;                  /* path that ran (QESMC_PATH_PLAIN / _SPLIT)         */
+  int path;                  /* path that ran (QESMC_PATH_PLAIN / _SPLIT / _FUSED) */
   int compressed;            /* 1 if the wall-mask stencil was used               */
   int status;                /* same as the return code                           */
   double final_max_dlambda;  /* max|dlambda| of the last iteration                */

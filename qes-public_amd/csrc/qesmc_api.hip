@@ -22,6 +22,7 @@ hipError_t qesmc_launch_divergence(qesmc_ctx *x, bool split, int alpha1);
 hipError_t qesmc_launch_stencil_encode(qesmc_ctx *x);
 hipError_t qesmc_launch_split_coefs(qesmc_ctx *x);
 hipError_t qesmc_launch_pack(qesmc_ctx *x, bool to_split);
+hipError_t qesmc_launch_unpack_selected(qesmc_ctx *x);
 hipError_t qesmc_launch_iteration(qesmc_ctx *x, int path, bool masked, int t, int itermax,
                                   float omega, float tol);
 hipError_t qesmc_launch_finish(qesmc_ctx *x, bool mirror, int alpha1, int alpha2);
@@ -81,6 +82,7 @@ void free_geometry(qesmc_ctx *x)
     dfree(x->d_lc[c]);
     dfree(x->d_Rc[c]);
     dfree(x->d_mask[c]);
+    dfree(x->d_lc2[c]);
     for (auto &p : x->d_cc[c]) dfree(p);
   }
   dfree(x->d_canon);
@@ -99,7 +101,11 @@ int alloc_geometry(qesmc_ctx *x, int nx, int ny, int nz)
   d.NY = ny - 1;
   d.NZ = nz - 1;
   const int half = (d.NX + 1) / 2;
-  d.HP = (half + 15) / 16 * 16;  // 64-byte aligned half-rows
+  // half-rows padded to a multiple of 4 half-cells: the split kernels' vector
+  // accesses (up to 4 half-cells per lane) then never straddle a row and stay
+  // 16-byte aligned; wider padding only adds traffic (a 400-cell row padded
+  // to 64-byte alignment stores 208 half-cells for 200)
+  d.HP = (half + 3) / 4 * 4;
   d.sy = d.NX;
   d.sz = (long long)d.NX * d.NY;
   d.fy = nx;
@@ -260,6 +266,7 @@ int qesmc_destroy(qesmc_ctx *x)
   dfree(x->d_err);
   dfree(x->d_status);
   dfree(x->d_norm);
+  dfree(x->d_cur);
   if (x->h_poll) (void)hipHostFree(x->h_poll);
   for (auto &ev : x->ev)
     if (ev) (void)hipEventDestroy(ev);
@@ -342,28 +349,32 @@ int run_prepare(qesmc_ctx *x, const qesmc_params *pp, RunState &st)
   p = *pp;
   if (p.itermax < 0) return QESMC_ERR_ARG;
   if (p.check_every < 1) p.check_every = 64;
-  if (p.path != QESMC_PATH_AUTO && p.path != QESMC_PATH_PLAIN && p.path != QESMC_PATH_SPLIT)
+  if (p.path != QESMC_PATH_AUTO && p.path != QESMC_PATH_PLAIN && p.path != QESMC_PATH_SPLIT &&
+      p.path != QESMC_PATH_FUSED)
     return QESMC_ERR_ARG;
   if (p.tile_k < 0 || p.wg_per_cu < 0) return QESMC_ERR_ARG;
   if (p.tile_v != 0 && p.tile_v != 1 && p.tile_v != 2 && p.tile_v != 4) return QESMC_ERR_ARG;
-  st.path = (p.path == QESMC_PATH_PLAIN) ? QESMC_PATH_PLAIN : QESMC_PATH_SPLIT;
+  st.path = (p.path == QESMC_PATH_PLAIN || p.path == QESMC_PATH_FUSED) ? p.path : QESMC_PATH_SPLIT;
   HIPCHK(hipSetDevice(x->device));
   if (p.flags & QESMC_RUN_REENCODE) {
     int rc = encode_stencil(x);
     if (rc != QESMC_OK) return rc;
   }
   st.masked = x->compressible;
+  if (st.path == QESMC_PATH_FUSED && !st.masked) return QESMC_ERR_UNSUPPORTED;
   x->kplanes = p.tile_k > 0 ? p.tile_k : x->kplanes_auto;
   x->tile_v = p.tile_v > 0 ? p.tile_v : x->tile_v_auto;
   if (x->kplanes == 0 || x->tile_v == 0) {
     // Untuned default: the most work per wave tile (planes x half-cells per
-    // lane) that still leaves >= 6 wave tiles per CU; small domains are
-    // latency-bound and need the parallelism more than long k-marches.
+    // lane, at most 4 x 2) that still leaves >= 6 wave tiles per CU; small
+    // domains are latency-bound and need the parallelism more than long
+    // k-marches.  Measured on MI355X: 4 x 2 matches 8 x 2 on 400x400x120 and
+    // 1000x1000x150 and is 25 % faster on 256x256x96.
     const long long flat = (long long)(x->d.NY - 2) * x->d.HP;
     auto tiles = [&](int kb, int vq) {
       return ((flat + 64LL * vq - 1) / (64LL * vq)) * ((x->d.NZ - 2 + kb - 1) / kb);
     };
-    static const int order[][2] = {{8, 2}, {4, 2}, {8, 1}, {2, 2}, {4, 1}, {2, 1}, {1, 2}, {1, 1}};
+    static const int order[][2] = {{4, 2}, {2, 2}, {4, 1}, {2, 1}, {1, 2}, {1, 1}};
     int bk = 0, bv = 0;
     long long most = -1;
     for (const auto &o : order) {
@@ -383,8 +394,19 @@ int run_prepare(qesmc_ctx *x, const qesmc_params *pp, RunState &st)
     if (!x->kplanes) x->kplanes = bk ? bk : 8;
     if (!x->tile_v) x->tile_v = bv ? bv : 1;
   }
-  // untuned: 7 resident work-groups per CU for two half-cells per lane, 8 for one
-  x->wg_per_cu = p.wg_per_cu > 0 ? p.wg_per_cu : x->wg_per_cu_auto > 0 ? x->wg_per_cu_auto : x->tile_v >= 2 ? 7 : 8;
+  // untuned: 4 resident work-groups per CU for two or four half-cells per
+  // lane, 5 for one (fewer, longer-lived work-groups beat full occupancy on
+  // every measured size but the HBM-bound 1000x1000x150, where they lose 1 %)
+  x->wg_per_cu = p.wg_per_cu > 0 ? p.wg_per_cu : x->wg_per_cu_auto > 0 ? x->wg_per_cu_auto : x->tile_v >= 2 ? 4 : 5;
+  if (st.path == QESMC_PATH_FUSED) {
+    // one work-group per 64 x 8 x tile_k tile (tile_v and wg_per_cu unused)
+    x->kplanes = p.tile_k > 0 ? p.tile_k : 8;
+    x->tile_v = 1;
+    x->wg_per_cu = 0;
+    for (int c = 0; c < 2; ++c)
+      if (!x->d_lc2[c]) HIPCHK(dalloc(&x->d_lc2[c], x->nsplit));
+    if (!x->d_cur) HIPCHK(dalloc(&x->d_cur, 1));
+  }
 
   const long long rows = (long long)p.itermax + 1;
   if (x->err_cap < rows) {
@@ -399,8 +421,15 @@ int run_prepare(qesmc_ctx *x, const qesmc_params *pp, RunState &st)
   HIPCHK(hipMemsetAsync(x->d_status, 0, sizeof(unsigned), s));
   HIPCHK(hipMemsetAsync(x->d_err, 0, sizeof(unsigned) * (size_t)(rows * kErrSlots), s));
   HIPCHK(qesmc_launch_set_u32(x, x->d_err, 0x3f800000u));  // row 0: 1.0f
-  if (st.path == QESMC_PATH_SPLIT) HIPCHK(qesmc_launch_pack(x, true));
-  HIPCHK(qesmc_launch_divergence(x, st.path == QESMC_PATH_SPLIT, p.alpha1));
+  if (st.path != QESMC_PATH_PLAIN) HIPCHK(qesmc_launch_pack(x, true));
+  if (st.path == QESMC_PATH_FUSED) {
+    // both buffer sets start equal: cells that are never relaxed keep their
+    // values in both, and iteration 0 reads set 0
+    for (int c = 0; c < 2; ++c)
+      HIPCHK(hipMemcpyAsync(x->d_lc2[c], x->d_lc[c], sizeof(float) * (size_t)x->nsplit, hipMemcpyDeviceToDevice, s));
+    HIPCHK(qesmc_launch_set_u32(x, x->d_cur, 0u));
+  }
+  HIPCHK(qesmc_launch_divergence(x, st.path != QESMC_PATH_PLAIN, p.alpha1));
   HIPCHK(hipEventRecord(x->ev[1], s));
   st.first_runs = p.itermax > 0 && (1.0f > p.tol);
   return QESMC_OK;
@@ -414,6 +443,7 @@ int run_finish(qesmc_ctx *x, const RunState &st, int launched, qesmc_result *res
   hipStream_t s = x->stream;
   HIPCHK(hipEventRecord(x->ev[2], s));
   if (st.path == QESMC_PATH_SPLIT) HIPCHK(qesmc_launch_pack(x, false));
+  if (st.path == QESMC_PATH_FUSED) HIPCHK(qesmc_launch_unpack_selected(x));
   HIPCHK(qesmc_launch_finish(x, st.first_runs, p.alpha1, p.alpha2));
   HIPCHK(hipEventRecord(x->ev[3], s));
   HIPCHK(hipStreamSynchronize(s));
@@ -433,7 +463,7 @@ int run_finish(qesmc_ctx *x, const RunState &st, int launched, qesmc_result *res
     (void)hipEventElapsedTime(&ms_sor, x->ev[1], x->ev[2]);
     res->iterations = count;
     res->path = st.path;
-    res->compressed = (st.path == QESMC_PATH_SPLIT && st.masked) ? 1 : 0;
+    res->compressed = (st.path != QESMC_PATH_PLAIN && st.masked) ? 1 : 0;
     res->status = rc;
     res->final_max_dlambda = last;
     res->ms_total = ms_total;

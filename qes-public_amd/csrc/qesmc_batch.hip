@@ -396,8 +396,9 @@ int qesmc_autotune(qesmc_ctx *x, const qesmc_params *base, int iterations, qesmc
   float *saved = nullptr;
   BCHK(hipMalloc((void **)&saved, sizeof(float) * (size_t)x->ncell));
   BCHK(hipMemcpyAsync(saved, x->d_lam, sizeof(float) * (size_t)x->ncell, hipMemcpyDeviceToDevice, x->stream));
-  static const int cand[][3] = {{8, 2, 8}, {8, 1, 8}, {4, 2, 8}, {4, 2, 4}, {4, 1, 8}, {2, 2, 8},
-                                {2, 1, 8}, {1, 2, 8}, {16, 1, 8}, {8, 2, 4}, {8, 1, 16}, {8, 4, 8}};
+  static const int cand[][3] = {{4, 2, 4}, {8, 2, 4}, {10, 2, 6}, {6, 2, 4}, {7, 2, 5},
+                                {3, 2, 4}, {10, 2, 8}, {6, 2, 8}, {2, 2, 4}, {4, 1, 5},
+                                {6, 1, 6}, {10, 1, 6}, {2, 1, 5}, {1, 2, 4}, {1, 1, 6}};
   double best_ms = 1e300;
   int bi = 0, rc = QESMC_OK;
   for (int c = 0; c < (int)(sizeof(cand) / sizeof(cand[0])); ++c) {

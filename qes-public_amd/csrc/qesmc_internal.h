@@ -77,6 +77,8 @@ struct qesmc_ctx {
   unsigned char *d_mask[2] = {};
   float *d_cc[2][6] = {};     // full coefficients, split (only if not compressible)
   unsigned *d_canon = nullptr; // 6*NZ wall values per layer (float bits)
+  float *d_lc2[2] = {};       // second lambda buffer set of the fused path (allocated on first use)
+  unsigned *d_cur = nullptr;   // fused path: buffer set holding the latest iterate (0: d_lc, 1: d_lc2)
 
   // convergence slots, kErrSlots words per iteration: row 0 holds the 1.0f
   // sentinel, row t+1 the striped max|dlambda| of iteration t

@@ -255,6 +255,24 @@ __global__ __launch_bounds__(256) void k_pack(Dims d, const float *__restrict__ 
   (((i + j + k) & 1) ? l1 : l0)[s] = lam[c];
 }
 
+// split -> natural from the buffer set the fused path left its last iterate in
+__global__ __launch_bounds__(256) void k_unpack_selected(Dims d, float *__restrict__ lam,
+                                                         const float *__restrict__ a0,
+                                                         const float *__restrict__ a1,
+                                                         const float *__restrict__ b0,
+                                                         const float *__restrict__ b1,
+                                                         const unsigned *__restrict__ cur)
+{
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y * 4 + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= d.NX || j >= d.NY) return;
+  const long long c = i + (long long)j * d.sy + k * d.sz;
+  const long long s = (long long)(k * d.NY + j) * d.HP + (i >> 1);
+  const bool odd = (i + j + k) & 1;
+  lam[c] = (*cur ? (odd ? b1 : b0) : (odd ? a1 : a0))[s];
+}
+
 __global__ __launch_bounds__(256) void k_unpack(Dims d, float *__restrict__ lam,
                                                 const float *__restrict__ l0,
                                                 const float *__restrict__ l1)
@@ -401,6 +419,7 @@ static const unsigned kPlaneBlocks = 8;  // work-groups per plane of the plane r
 
 using namespace qesmc;
 
+hipError_t qesmc_launch_fused_iteration(qesmc_ctx *x, int t, int itermax, float omega, float tol);
 hipError_t qesmc_launch_split_iteration(qesmc_ctx *x, bool masked, int t, int itermax, float omega,
                                         float tol);
 
@@ -456,6 +475,14 @@ hipError_t qesmc_launch_pack(qesmc_ctx *x, bool to_split)
   return hipGetLastError();
 }
 
+hipError_t qesmc_launch_unpack_selected(qesmc_ctx *x)
+{
+  const Dims &d = x->d;
+  hipLaunchKernelGGL(k_unpack_selected, grid_xy(d.NX, d.NY, d.NZ), kBlock, 0, x->stream, d, x->d_lam, x->d_lc[0],
+                     x->d_lc[1], x->d_lc2[0], x->d_lc2[1], x->d_cur);
+  return hipGetLastError();
+}
+
 hipError_t qesmc_launch_iteration(qesmc_ctx *x, int path, bool masked, int t, int itermax,
                                   float omega, float tol)
 {
@@ -469,6 +496,8 @@ hipError_t qesmc_launch_iteration(qesmc_ctx *x, int path, bool masked, int t, in
       hipLaunchKernelGGL(k_rb_plain, gr, kBlock, 0, x->stream, d, colour, kp, x->d_coef[0],
                          x->d_coef[1], x->d_coef[2], x->d_coef[3], x->d_coef[4], x->d_coef[5],
                          x->d_R, x->d_lam, omega, omo, tol, x->d_err, t, itermax, x->d_status);
+  } else if (path == QESMC_PATH_FUSED) {
+    return qesmc_launch_fused_iteration(x, t, itermax, omega, tol);
   } else {
     return qesmc_launch_split_iteration(x, masked, t, itermax, omega, tol);
   }

@@ -68,6 +68,18 @@ struct Vec<4> {
   }
 };
 
+// The relaxation of one cell, operand for operand the reference's
+// (Solver_CPU_RB.cpp:117-124): xp, xm, yp, ym, zp, zm are lambda at i+1, i-1,
+// j+1, j-1, k+1, k-1.  Shared by every split-layout kernel so that all of
+// them round identically.
+__device__ __forceinline__ float relax(float ec, float fc, float gc, float hc, float mc, float nc, float xp,
+                                       float xm, float yp, float ym, float zp, float zm, float r, float o,
+                                       float omega, float one_m_omega)
+{
+  return (omega / (ec + fc + gc + hc + mc + nc)) * (ec * xp + fc * xm + gc * yp + hc * ym + mc * zp + nc * zm - r)
+         + one_m_omega * o;
+}
+
 template <bool MASKED, int VQ>
 __device__ __forceinline__ float sor_tile(const Dims &d, int colour, int t, int j, int q, int k0, int k1,
                                           float *__restrict__ own, const float *__restrict__ oth,
@@ -125,9 +137,8 @@ __device__ __forceinline__ float sor_tile(const Dims &d, int colour, int t, int 
         mc = upd[v] ? cm[x + v] : 0.f;
         nc = upd[v] ? cn[x + v] : 0.f;
       }
-      const float nv = (omega / (ec + fc + gc + hc + mc + nc))
-                           * (ec * xp + fc * xm + gc * yp[v] + hc * ym[v] + mc * zp[v] + nc * zm[v] - r[v])
-                       + one_m_omega * o[v];
+      const float nv = relax(ec, fc, gc, hc, mc, nc, xp, xm, yp[v], ym[v], zp[v], zm[v], r[v], o[v], omega,
+                             one_m_omega);
       nvv[v] = nv;
       if (upd[v]) {
         if (!isfinite(nv)) atomicOr(status, ST_NONFINITE);

@@ -31,6 +31,7 @@ ERR_UNSUPPORTED = -8
 PATH_AUTO = 0
 PATH_PLAIN = 1
 PATH_SPLIT = 2
+PATH_FUSED = 3
 RUN_REENCODE = 1
 
 # every symbol include/qesmc.h declares

@@ -14,7 +14,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-PLAIN, SPLIT = 1, 2
+PLAIN, SPLIT, FUSED = 1, 2, 3
 
 
 def single_cube_city(O):
@@ -56,18 +56,18 @@ def check_solve(ctx, O, p, path, lam0=None, **kw):
     return res
 
 
-@pytest.mark.parametrize("path", [PLAIN, SPLIT])
+@pytest.mark.parametrize("path", [PLAIN, SPLIT, FUSED])
 def test_single_cube_100x100x50_reference_schedule_bit_exact(gpu_ctx, oracle, path):
     p = single_cube_city(oracle)
     res = check_solve(gpu_ctx, oracle, p, path)
     assert res["iterations"] == 500 and res["path"] == path
-    if path == SPLIT:
+    if path != PLAIN:
         assert res["compressed"] == 1
 
 
 @pytest.mark.parametrize("shape", [(2, 2, 1), (3, 3, 1), (5, 4, 2), (7, 5, 3), (13, 11, 6), (65, 33, 9),
                                    (130, 67, 17), (201, 3, 4)])
-@pytest.mark.parametrize("path", [PLAIN, SPLIT])
+@pytest.mark.parametrize("path", [PLAIN, SPLIT, FUSED])
 def test_ragged_shapes_bit_exact(gpu_ctx, oracle, qesmc, shape, path):
     ncx, ncy, ncz = shape
     seed = ncx * 31 + ncy * 7 + ncz
@@ -96,7 +96,7 @@ def test_tolerance_stop_matches_oracle(gpu_ctx, oracle, qesmc, check_every):
         lam, it, me = oracle.sor_rb_f32(p, R, lam, itermax=1, tol=0.0)
         errs.append(me)
     tol = float(np.float32(errs[17]))
-    for path in (PLAIN, SPLIT):
+    for path in (PLAIN, SPLIT, FUSED):
         res = check_solve(gpu_ctx, oracle, p, path, itermax=400, tol=tol, check_every=check_every)
         assert res["iterations"] == 18
 
@@ -105,7 +105,7 @@ def test_tolerance_stop_matches_oracle(gpu_ctx, oracle, qesmc, check_every):
 def test_no_iteration_when_initial_error_not_above_tol(gpu_ctx, oracle, qesmc, tol):
     p = random_city(oracle, qesmc, 8, 20, 18, 6)
     lam0 = np.random.default_rng(2).standard_normal(p.ncell).astype(np.float32)
-    for path in (PLAIN, SPLIT):
+    for path in (PLAIN, SPLIT, FUSED):
         res = check_solve(gpu_ctx, oracle, p, path, lam0=lam0, itermax=50, tol=tol)
         assert res["iterations"] == 0
 
@@ -121,6 +121,10 @@ def test_fractional_coefficients_use_full_stencil(gpu_ctx, oracle, qesmc):
     res = check_solve(gpu_ctx, oracle, p, SPLIT, itermax=80)
     assert res["compressed"] == 0
     check_solve(gpu_ctx, oracle, p, PLAIN, itermax=80)
+    # the fused path has no full-coefficient variant: it refuses
+    with pytest.raises(qesmc.QesmcError) as ei:
+        gpu_ctx.solve(p, path=FUSED, itermax=5)
+    assert ei.value.code == qesmc.ERR_UNSUPPORTED
     # back to a stair-step geometry on the same context: mask path again
     q = random_city(oracle, qesmc, 34, 60, 44, 12)
     assert check_solve(gpu_ctx, oracle, q, SPLIT, itermax=20)["compressed"] == 1
@@ -213,7 +217,7 @@ def test_city_400x400x120_reference_schedule(gpu_ctx, oracle, qesmc):
     u_ref = 0.5 / 0.4 * math.log((10.0 + 0.5) / 0.5)
     p = oracle.build_city(ncx, ncy, dx, dz, blds, (1, u_ref, 10.0, 0.5, 217.0))
     ou, ov, ow, olam, oit, ome = oracle.solve_rb_f32(p, itermax=500)
-    for path in (SPLIT, PLAIN):
+    for path in (FUSED, SPLIT, PLAIN):
         u, v, w, lam, res = gpu_ctx.solve(p, path=path)
         assert res["iterations"] == oit == 500
         assert_same(lam, olam, f"lambda path {path}")
@@ -230,3 +234,4 @@ def test_tile_parameters_never_change_results(gpu_ctx, oracle, qesmc, tile):
     res = check_solve(gpu_ctx, oracle, p, SPLIT, itermax=40, tile_k=tk, tile_v=tv, wg_per_cu=wg)
     assert res["compressed"] == 1
     check_solve(gpu_ctx, oracle, p, PLAIN, itermax=12, tile_k=tk)
+    assert check_solve(gpu_ctx, oracle, p, FUSED, itermax=40, tile_k=tk)["tile_k"] == tk
