@@ -431,6 +431,69 @@ __global__ __launch_bounds__(256) void k_up_bwd(const bf16_t *gout, int ldo, int
   gin[vp * ldi + c] = f2bf(s);
 }
 
+// The same three data-movement steps with 8 channels (16 bytes) per thread
+// and 32-bit index math, for channel counts, strides and offsets that are
+// multiples of 8 and V * ch / 8 < 2^32 (the host checks): same arithmetic in
+// the same order per channel, so the same results.
+__device__ inline float bf_at(const uint4 &t, int e)
+{
+  const unsigned w = e < 2 ? t.x : e < 4 ? t.y : e < 6 ? t.z : t.w;
+  return __uint_as_float((e & 1) ? (w & 0xffff0000u) : (w << 16));
+}
+
+__global__ __launch_bounds__(256) void k_pool_fwd8(const bf16_t *in, int ldi, int ioff, bf16_t *out, int ldo, int ch,
+                                                   int odx, int ody, int odz)
+{
+  const unsigned g = (unsigned)ch >> 3;
+  const unsigned q = blockIdx.x * 256u + threadIdx.x;
+  if (q >= (unsigned)odx * ody * odz * g) return;
+  const unsigned vo = q / g;
+  const unsigned c8 = (q - vo * g) * 8u;
+  const unsigned x = vo % odx, y = (vo / odx) % ody, z = vo / ((unsigned)odx * ody);
+  const unsigned idx = 2u * odx, idy = 2u * ody;
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const unsigned vi = ((2u * z + (b >> 2)) * idy + (2u * y + ((b >> 1) & 1))) * idx + 2u * x + (b & 1);
+    const uint4 t = *reinterpret_cast<const uint4 *>(in + (size_t)vi * ldi + ioff + c8);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) s[e] += bf_at(t, e);
+  }
+  unsigned o[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e)
+    o[e] = (unsigned)f2bf(s[2 * e] * 0.125f) | ((unsigned)f2bf(s[2 * e + 1] * 0.125f) << 16);
+  *reinterpret_cast<uint4 *>(out + (size_t)vo * ldo + c8) = make_uint4(o[0], o[1], o[2], o[3]);
+}
+
+__global__ __launch_bounds__(256) void k_up_fwd8(const bf16_t *in, int ldi, bf16_t *out, int ldo, int ooff, int ch,
+                                                 int dx, int dy, int dz)
+{
+  const unsigned g = (unsigned)ch >> 3;
+  const unsigned q = blockIdx.x * 256u + threadIdx.x;
+  if (q >= (unsigned)dx * dy * dz * g) return;
+  const unsigned v = q / g;
+  const unsigned c8 = (q - v * g) * 8u;
+  const unsigned x = v % dx, y = (v / dx) % dy, z = v / ((unsigned)dx * dy);
+  const unsigned vp = ((z >> 1) * ((unsigned)dy >> 1) + (y >> 1)) * ((unsigned)dx >> 1) + (x >> 1);
+  *reinterpret_cast<uint4 *>(out + (size_t)v * ldo + ooff + c8) =
+      *reinterpret_cast<const uint4 *>(in + (size_t)vp * ldi + c8);
+}
+
+__global__ __launch_bounds__(256) void k_head_fwd8(const bf16_t *x, int ldx, int ch, const float *w, float *y,
+                                                   long long V)
+{
+  const long long v = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (v >= V) return;
+  float s = w[ch];
+  for (int c8 = 0; c8 < ch; c8 += 8) {
+    const uint4 t = *reinterpret_cast<const uint4 *>(x + v * ldx + c8);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) s += w[c8 + e] * bf_at(t, e);
+  }
+  y[v] = s;
+}
+
 // head: y[v] = sum_c W[c] x[v][c] + b
 __global__ __launch_bounds__(256) void k_head_fwd(const bf16_t *x, int ldx, int ch, const float *w, float *y,
                                                   long long V)
@@ -768,6 +831,13 @@ hipError_t dmalloc(T **p, long long n)
 
 inline dim3 g1(long long n) { return dim3((unsigned)((n + 255) / 256)); }
 
+// the 8-channel data-movement kernels apply: channels, strides and offset in
+// whole 16-byte groups, and every index fits 32 bits
+inline bool vec8(int ch, int ld_a, int ld_b, int off, long long V)
+{
+  return ch % 8 == 0 && ld_a % 8 == 0 && ld_b % 8 == 0 && off % 8 == 0 && V * (long long)(ld_a > ld_b ? ld_a : ld_b) < (1LL << 31);
+}
+
 hipError_t launch_conv_fwd(hipStream_t s, const ConvFwdArgs &a, int cout)
 {
   const long long V = (long long)a.dx * a.dy * a.dz;
@@ -917,8 +987,12 @@ hipError_t forward(qesmc_unet *u, const bf16_t *feat)
     if ((e = conv_forward(u, u->enc2[l], lv, lv.A, lv.c, lv.CAT, lv.ldcat, 0)) != hipSuccess) return e;
     if (l + 1 < L) {
       Level &nx = u->lv[l + 1];
-      hipLaunchKernelGGL(k_pool_fwd, g1(nx.V * lv.c), dim3(256), 0, s, lv.CAT, lv.ldcat, 0, nx.P, lv.c, lv.c, nx.dx,
-                         nx.dy, nx.dz);
+      if (vec8(lv.c, lv.ldcat, lv.c, 0, nx.V))
+        hipLaunchKernelGGL(k_pool_fwd8, g1(nx.V * lv.c / 8), dim3(256), 0, s, lv.CAT, lv.ldcat, 0, nx.P, lv.c, lv.c,
+                           nx.dx, nx.dy, nx.dz);
+      else
+        hipLaunchKernelGGL(k_pool_fwd, g1(nx.V * lv.c), dim3(256), 0, s, lv.CAT, lv.ldcat, 0, nx.P, lv.c, lv.c,
+                           nx.dx, nx.dy, nx.dz);
     }
   }
   for (int l = L - 2; l >= 0; --l) {
@@ -926,14 +1000,22 @@ hipError_t forward(qesmc_unet *u, const bf16_t *feat)
     Level &dn = u->lv[l + 1];
     const bf16_t *src = (l == L - 2) ? dn.CAT : dn.D;
     const int ldsrc = (l == L - 2) ? dn.ldcat : dn.c;
-    hipLaunchKernelGGL(k_up_fwd, g1(lv.V * dn.c), dim3(256), 0, s, src, ldsrc, lv.CAT, lv.ldcat, lv.c, dn.c, lv.dx,
-                       lv.dy, lv.dz);
+    if (vec8(dn.c, ldsrc, lv.ldcat, lv.c, lv.V))
+      hipLaunchKernelGGL(k_up_fwd8, g1(lv.V * dn.c / 8), dim3(256), 0, s, src, ldsrc, lv.CAT, lv.ldcat, lv.c, dn.c,
+                         lv.dx, lv.dy, lv.dz);
+    else
+      hipLaunchKernelGGL(k_up_fwd, g1(lv.V * dn.c), dim3(256), 0, s, src, ldsrc, lv.CAT, lv.ldcat, lv.c, dn.c, lv.dx,
+                         lv.dy, lv.dz);
     if ((e = conv_forward(u, u->dec1[l], lv, lv.CAT, lv.ldcat, lv.E, lv.c, 0)) != hipSuccess) return e;
     if ((e = conv_forward(u, u->dec2[l], lv, lv.E, lv.c, lv.D, lv.c, 0)) != hipSuccess) return e;
   }
   Level &top = u->lv[0];
-  hipLaunchKernelGGL(k_head_fwd, g1(top.V), dim3(256), 0, s, top.D, top.c, top.c, u->p + u->head_off, u->out,
-                     top.V);
+  if (vec8(top.c, top.c, top.c, 0, top.V))
+    hipLaunchKernelGGL(k_head_fwd8, g1(top.V), dim3(256), 0, s, top.D, top.c, top.c, u->p + u->head_off, u->out,
+                       top.V);
+  else
+    hipLaunchKernelGGL(k_head_fwd, g1(top.V), dim3(256), 0, s, top.D, top.c, top.c, u->p + u->head_off, u->out,
+                       top.V);
   return hipGetLastError();
 }
 
