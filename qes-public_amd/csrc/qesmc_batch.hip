@@ -56,7 +56,7 @@ __device__ inline float wave_max_b(float v)
 // wave's tile indices only increase, so its instance is found by advancing
 // from the previous one; the instance's gate is read and its max|dlambda| is
 // published once per instance the wave visits, not per tile.
-template <bool ALL_MASKED>
+template <bool ALL_MASKED, int VQ>
 __global__ __launch_bounds__(256) void k_sor_batch(const BatchItem *__restrict__ items, int n, long long total,
                                                    int colour, const int *t_base, int t_off, int itermax,
                                                    float omega, float one_m_omega, float tol)
@@ -130,7 +130,7 @@ __global__ __launch_bounds__(256) void k_sor_batch(const BatchItem *__restrict__
     const long long lt = wt - cbeg;
     const int kc = (int)(lt / tpp);
     const int pt = (int)(lt - (long long)kc * tpp);
-    const int p = pt * 64 + lane;
+    const int p = pt * (64 * VQ) + lane * VQ;
     const int jr = p / d.HP;
     const int j = jr + 1;
     const int q = p - jr * d.HP;
@@ -144,11 +144,11 @@ __global__ __launch_bounds__(256) void k_sor_batch(const BatchItem *__restrict__
     const unsigned char *mk_ = (const unsigned char *)mk;
     const unsigned *canon_ = (const unsigned *)canon;
     if (masked) {
-      dl = sor_tile<true, 1>(d, colour, t, j, q, k0, k1, o_, oth_, Rc_, mk_, canon_, nullptr, nullptr, nullptr,
-                             nullptr, nullptr, nullptr, omega, one_m_omega, status, dl);
+      dl = sor_tile<true, VQ>(d, colour, t, j, q, k0, k1, o_, oth_, Rc_, mk_, canon_, nullptr, nullptr, nullptr,
+                              nullptr, nullptr, nullptr, omega, one_m_omega, status, dl);
     } else if (!ALL_MASKED) {
       const float *const *cc = items[cur].cc[colour];
-      dl = sor_tile<false, 1>(d, colour, t, j, q, k0, k1, o_, oth_, Rc_, mk_, canon_,
+      dl = sor_tile<false, VQ>(d, colour, t, j, q, k0, k1, o_, oth_, Rc_, mk_, canon_,
                               (const float *)as_global(cc[0]), (const float *)as_global(cc[1]),
                               (const float *)as_global(cc[2]), (const float *)as_global(cc[3]),
                               (const float *)as_global(cc[4]), (const float *)as_global(cc[5]), omega, one_m_omega,
@@ -231,8 +231,13 @@ int qesmc_run_batch(qesmc_ctx **ctxs, int n, const qesmc_params *params, qesmc_r
     for (int j = 0; j < i; ++j)
       if (ctxs[j] == ctxs[i]) return QESMC_ERR_ARG;
   }
-  if (params->path == QESMC_PATH_PLAIN) return QESMC_ERR_UNSUPPORTED;
-  if (params->tile_v > 1) return QESMC_ERR_UNSUPPORTED;  // the batched kernel relaxes one half-cell per lane
+  if (params->path == QESMC_PATH_PLAIN || params->path == QESMC_PATH_FUSED) return QESMC_ERR_UNSUPPORTED;
+  // half-cells per lane, one value for the whole batch: 1 or 2 (default 2)
+  const int vq = params->tile_v == 0 ? 2 : params->tile_v;
+  if (vq != 1 && vq != 2) return QESMC_ERR_UNSUPPORTED;
+  qesmc_params pv = *params;
+  pv.tile_v = vq;  // each instance picks its tile_k for this tile_v
+  params = &pv;
   qesmc_ctx *lead = ctxs[0];
   BCHK(hipSetDevice(lead->device));
   hipStream_t bs = lead->stream;
@@ -274,7 +279,7 @@ int qesmc_run_batch(qesmc_ctx **ctxs, int n, const qesmc_params *params, qesmc_r
     it.masked = st[i].masked ? 1 : 0;
     it.kb = x->kplanes < 1 ? 1 : x->kplanes;
     const long long flat = (long long)(x->d.NY - 2) * x->d.HP;
-    it.tiles_per_plane = (int)((flat + 63) / 64);
+    it.tiles_per_plane = (int)((flat + 64LL * vq - 1) / (64LL * vq));
     const int nkc = x->d.NZ - 2 >= 1 && x->d.NY - 2 >= 1 ? (x->d.NZ - 2 + it.kb - 1) / it.kb : 0;
     it.tile_begin = total;
     total += (long long)it.tiles_per_plane * nkc;
@@ -316,9 +321,11 @@ int qesmc_run_batch(qesmc_ctx **ctxs, int n, const qesmc_params *params, qesmc_r
   int wg = p.wg_per_cu > 0 ? p.wg_per_cu : lead->wg_per_cu_auto > 0 ? lead->wg_per_cu_auto : 8;
   {
     int fit = 0;
-    const hipError_t oe = all_masked
-        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, k_sor_batch<true>, 256, 0)
-        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, k_sor_batch<false>, 256, 0);
+    const hipError_t oe =
+        all_masked ? (vq == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, k_sor_batch<true, 2>, 256, 0)
+                              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, k_sor_batch<true, 1>, 256, 0))
+                   : (vq == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, k_sor_batch<false, 2>, 256, 0)
+                              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, k_sor_batch<false, 1>, 256, 0));
     if (oe == hipSuccess && fit > 0 && fit < wg) wg = fit;
   }
   long long blocks = (total + 3) / 4;
@@ -333,12 +340,17 @@ int qesmc_run_batch(qesmc_ctx **ctxs, int n, const qesmc_params *params, qesmc_r
   if (any_runs) {
     auto launch_iter = [&](const int *tb, int toff) {
       for (int colour = 0; colour < 2; ++colour) {
-        if (all_masked)
-          hipLaunchKernelGGL(k_sor_batch<true>, dim3((unsigned)blocks), dim3(256), 0, bs, d_items, n, total, colour,
-                             tb, toff, p.itermax, p.omega, omo, p.tol);
-        else
-          hipLaunchKernelGGL(k_sor_batch<false>, dim3((unsigned)blocks), dim3(256), 0, bs, d_items, n, total, colour,
-                             tb, toff, p.itermax, p.omega, omo, p.tol);
+#define QESMC_BATCH_LAUNCH(M, V)                                                                           \
+  hipLaunchKernelGGL((k_sor_batch<M, V>), dim3((unsigned)blocks), dim3(256), 0, bs, d_items, n, total, colour, \
+                     tb, toff, p.itermax, p.omega, omo, p.tol)
+        if (all_masked) {
+          if (vq == 2) QESMC_BATCH_LAUNCH(true, 2);
+          else QESMC_BATCH_LAUNCH(true, 1);
+        } else {
+          if (vq == 2) QESMC_BATCH_LAUNCH(false, 2);
+          else QESMC_BATCH_LAUNCH(false, 1);
+        }
+#undef QESMC_BATCH_LAUNCH
       }
     };
     if (use_graph) {

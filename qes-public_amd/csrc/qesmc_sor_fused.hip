@@ -279,7 +279,6 @@ __global__ __launch_bounds__(256) void k_sor_fused(Dims d, int kb, int qt, int j
   const int j0 = 1 + tj * kTj;
   const int k0 = 1 + kc * kb;
   const int k1 = k0 + kb < d.NZ - 1 ? k0 + kb : d.NZ - 1;  // black planes [k0, k1)
-  const long long HP = d.HP, sp = d.splane;
 
   float dl = 0.f;
   bool bad = false;
@@ -287,8 +286,8 @@ __global__ __launch_bounds__(256) void k_sor_fused(Dims d, int kb, int qt, int j
   const int pend = k1;      // <= NZ-1
 
   // One plane step at a time.  Prefetching step p + 1 during step p (two
-  // load buffers) was measured slower: the extra registers cut occupancy
-  // from 5 to 3 waves per SIMD, and this kernel needs the waves more.
+  // load buffers) was measured slower: the extra registers cut occupancy,
+  // and this kernel needs the waves more.
 #pragma unroll 1
   for (int p = pbeg; p <= pend; ++p) {
     const StepIn L = load_step(d, p, k0, j0, q0, wave, lane, a0, a1, R0, R1, mask0, mask1, canon);

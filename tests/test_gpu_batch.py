@@ -23,9 +23,11 @@ SHAPES = [(40, 32, 12), (7, 5, 3), (64, 48, 20), (3, 3, 1), (33, 65, 9), (90, 20
 
 @pytest.mark.parametrize("use_graph", [False, True])
 @pytest.mark.parametrize("tol", [1e-9, 3e-3])
-def test_batch_equals_oracle_per_problem(qesmc, oracle, use_graph, tol):
+@pytest.mark.parametrize("tile_v", [0, 1])
+def test_batch_equals_oracle_per_problem(qesmc, oracle, use_graph, tol, tile_v):
     """Uneven sizes, one problem with fractional (unmasked) coefficients, a
-    tolerance that stops problems at different iterations."""
+    tolerance that stops problems at different iterations; two or one
+    half-cells per lane."""
     probs = [city(oracle, qesmc, 200 + q, *s, fractional=(q == 2)) for q, s in enumerate(SHAPES)]
     ctxs = [qesmc.Context(0) for _ in probs]
     try:
@@ -33,7 +35,8 @@ def test_batch_equals_oracle_per_problem(qesmc, oracle, use_graph, tol):
             c.set_geometry(p)
             c.set_inflow(p.u0, p.v0, p.w0)
             c.set_lambda(None)
-        res, ms = qesmc.run_batch(ctxs, itermax=150, tol=tol, check_every=16, use_graph=use_graph)
+        res, ms = qesmc.run_batch(ctxs, itermax=150, tol=tol, check_every=16, use_graph=use_graph, tile_v=tile_v)
+        assert all(r["tile_v"] == (tile_v or 2) for r in res)
         iters = set()
         for q, (c, p, r) in enumerate(zip(ctxs, probs, res)):
             ou, ov, ow, olam, oit, ome = oracle.solve_rb_f32(p, itermax=150, tol=tol)
