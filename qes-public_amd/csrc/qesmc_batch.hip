@@ -317,8 +317,11 @@ int qesmc_run_batch(qesmc_ctx **ctxs, int n, const qesmc_params *params, qesmc_r
 
   bool all_masked = true;
   for (int i = 0; i < n; ++i) all_masked = all_masked && st[i].masked;
-  // persistent grid: never more work-groups than can be resident at once
-  int wg = p.wg_per_cu > 0 ? p.wg_per_cu : lead->wg_per_cu_auto > 0 ? lead->wg_per_cu_auto : 8;
+  // persistent grid: never more work-groups than can be resident at once;
+  // untuned, 4 per CU for two half-cells per lane and 5 for one, as for the
+  // single-problem kernel (8 was measured 10-15 % slower on a batch of two
+  // 400 x 400 x 120 cities)
+  int wg = p.wg_per_cu > 0 ? p.wg_per_cu : lead->wg_per_cu_auto > 0 ? lead->wg_per_cu_auto : vq == 2 ? 4 : 5;
   {
     int fit = 0;
     const hipError_t oe =

@@ -93,6 +93,44 @@ __global__ __launch_bounds__(256) void k_conv_fwd(ConvFwdArgs a)
     ++tap;
   }
   const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
+  if (a.cin % 32 == 0) {
+    // Each 32-wide K step lies inside one tap: taps outer, channel chunks
+    // inner, same K order as below.  The A loads read a clamped in-bounds
+    // address and the zero padding is selected after the load, so the loads
+    // of several K steps can be in flight together (the small tiles of the
+    // deep levels run long K loops with one output tile per wave).
+    constexpr int kTapUnroll = MT * NT <= 2 ? 3 : 1;
+#pragma unroll kTapUnroll
+    for (int tap = 0; tap < 27; ++tap) {
+      const int ox = tap % 3 - 1, oy = (tap / 3) % 3 - 1, oz = tap / 9 - 1;
+      long long nvt[MT];
+      bool okt[MT];
+#pragma unroll
+      for (int t = 0; t < MT; ++t) {
+        const int nx = vx[t] + ox, ny = vy[t] + oy, nz = vz[t] + oz;
+        okt[t] = vin[t] && nx >= 0 && nx < a.dx && ny >= 0 && ny < a.dy && nz >= 0 && nz < a.dz;
+        nvt[t] = okt[t] ? ((long long)nz * a.dy + ny) * a.dx + nx : 0;
+      }
+      for (int c0 = 0; c0 < a.cin; c0 += 32) {
+        const int cc = c0 + 8 * oct;
+        const int koct = tap * a.cin + cc;
+        bf16x8 av[MT];
+#pragma unroll
+        for (int t = 0; t < MT; ++t) {
+          const uint4 raw = *reinterpret_cast<const uint4 *>(a.x + nvt[t] * a.ldx + cc);
+          av[t] = as_bf16x8(okt[t] ? raw : zero4);
+        }
+#pragma unroll
+        for (int n = 0; n < NT; ++n) {
+          const bf16x8 bv =
+              as_bf16x8(*reinterpret_cast<const uint4 *>(a.w + (long long)(16 * (nb + n) + r) * a.kp + koct));
+#pragma unroll
+          for (int t = 0; t < MT; ++t)
+            acc[t][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[t], bv, acc[t][n], 0, 0, 0);
+        }
+      }
+    }
+  } else
   for (int kc = 0; kc < a.kp; kc += 32) {
     const int koct = kc + 8 * oct;
     const bool kvalid = koct < a.ktot;
