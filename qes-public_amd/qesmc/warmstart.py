@@ -119,6 +119,8 @@ def _main(argv=None):
     tr.add_argument("--samples", type=int, default=2000)
     tr.add_argument("--epochs", type=float, default=4.0)
     tr.add_argument("--lr", type=float, default=2e-3)
+    tr.add_argument("--lr-min", type=float, default=2e-4, help="end of the cosine learning-rate schedule")
+    tr.add_argument("--init", default=None, help="start from these weights (a file written by this command)")
     tr.add_argument("--size", type=int, nargs=3, default=[128, 128, 64])
     tr.add_argument("--solve-iters", type=int, default=2500)
     tr.add_argument("--smooth-block", type=int, default=8, help="block edge of the block-mean loss term")
@@ -135,16 +137,22 @@ def _main(argv=None):
 
     if a.cmd == "train":
         ncx, ncy, ncz = a.size
-        net = UNet(ncx, ncy, ncz, levels=4, base_channels=16, seed=1)
+        z = np.load(a.init) if a.init else None
+        # continuing from saved weights keeps their output unit
+        net = UNet(ncx, ncy, ncz, levels=4, base_channels=16, seed=1,
+                   target_scale=float(z["target_scale"]) if z is not None else 0.0)
+        if z is not None:
+            net.set_params(z["params"])
         if a.smooth_weight > 0:
             net.set_loss(a.smooth_block, a.smooth_weight)
         with Context(0) as ctx:
             t = build_dataset(net, ctx, range(a.samples), iters=a.solve_iters, tol=1e-9, log=log)
             log(f"dataset: {a.samples} cities in {t:.1f} s")
-            loss, ms = train(net, a.epochs, lr=a.lr, log=log)
+            loss, ms = train(net, a.epochs, lr=a.lr, lr_min=a.lr_min, log=log)
             log(f"training: {int(a.epochs * a.samples)} steps, {ms / 1e3:.1f} s on the device, last loss {loss:.4e}")
         save(net, a.out, samples=a.samples, epochs=a.epochs, final_loss=loss, train_device_s=ms / 1e3,
-             smooth_block=a.smooth_block, smooth_weight=a.smooth_weight)
+             smooth_block=a.smooth_block, smooth_weight=a.smooth_weight, lr=a.lr, lr_min=a.lr_min,
+             init=a.init or "")
         log(f"saved {a.out}")
         return 0
     net = load(a.weights)
