@@ -17,10 +17,10 @@
 // up the planes, it first relaxes the red cells of plane p over the tile
 // plus a halo (128 x 10: two half-cells and one row each side, recomputed
 // from A exactly as their owner does) into a 3-plane LDS ring, then the
-// black cells of plane p-1, whose six red neighbours are all in the ring.  Per iteration a cell costs
-// its own lambda read and write, R, the mask and one read of the other
-// colour (as a neighbour of a red cell): 13 bytes, against 17 per colour
-// pass when each pass is its own launch.  Tiles are numbered q-fastest, then
+// black cells of plane p-1, whose six red neighbours are all in the ring.
+// Per iteration a cell costs its own lambda read and write, R, the mask and
+// one read of the other colour (as a neighbour of a red cell): 13 bytes,
+// against 17 per colour pass when each pass is its own launch.  Tiles are numbered q-fastest, then
 // j, then plane chunk, and XCD x (work-group id % 8) takes the x-th eighth
 // of them, so the halo rows and columns a work-group re-reads sit in the L2
 // of the work-groups that own them.
@@ -39,7 +39,7 @@ constexpr int kRh = kTj + 2;     // ring rows (halo row each side)
 
 // Work-group barrier that orders LDS only: __syncthreads() also fences
 // global memory, which makes every wave wait for all of its outstanding
-// global loads (the prefetched next step) and stores at each barrier.
+// global loads and stores at each barrier.
 __device__ __forceinline__ void lds_barrier()
 {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
