@@ -77,15 +77,17 @@ __global__ __launch_bounds__(256) void k_vel_grad(Dims d, const int *__restrict_
         gzz = (w[f + fz] - w[f]) / (q.w);
       }
     }
-    G[c] = gxx;
-    G[c + ncell] = gxy;
-    G[c + 2 * ncell] = gxz;
-    G[c + 3 * ncell] = gyx;
-    G[c + 4 * ncell] = gyy;
-    G[c + 5 * ncell] = gyz;
-    G[c + 6 * ncell] = gzx;
-    G[c + 7 * ncell] = gzy;
-    G[c + 8 * ncell] = gzz;
+    // nine output streams that this kernel never reads back: non-temporal
+    // stores keep them from displacing the face rows still to be read
+    __builtin_nontemporal_store(gxx, G + c);
+    __builtin_nontemporal_store(gxy, G + c + ncell);
+    __builtin_nontemporal_store(gxz, G + c + 2 * ncell);
+    __builtin_nontemporal_store(gyx, G + c + 3 * ncell);
+    __builtin_nontemporal_store(gyy, G + c + 4 * ncell);
+    __builtin_nontemporal_store(gyz, G + c + 5 * ncell);
+    __builtin_nontemporal_store(gzx, G + c + 6 * ncell);
+    __builtin_nontemporal_store(gzy, G + c + 7 * ncell);
+    __builtin_nontemporal_store(gzz, G + c + 8 * ncell);
   }
 }
 
