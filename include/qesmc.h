@@ -38,6 +38,8 @@ extern "C" {
 #define QESMC_ERR_NONFINITE (-6)    /* lambda became NaN/Inf during the solve   */
 #define QESMC_ERR_STATE (-7)        /* resident call before its inputs were set */
 #define QESMC_ERR_UNSUPPORTED (-8)  /* option not available for these inputs   */
+#define QESMC_ERR_IO (-9)           /* file missing, unwritable or not a NetCDF
+                                       classic (CDF-1/CDF-2) file               */
 
 /* ---- grid / input / output descriptors ---------------------------------- */
 
@@ -256,6 +258,31 @@ int qesmc_lambda_combine(qesmc_ctx *ctx, int n, qesmc_ctx *const *srcs, const fl
 /* *out = max over all faces of |u_a - u_b|, |v_a - v_b|, |w_a - w_b| (FP64;
  * a NaN difference gives +inf).  Contexts on one device, same grid. */
 int qesmc_velocity_linf_diff(qesmc_ctx *a, qesmc_ctx *b, double *out);
+
+/* ---- QES workspace files (NetCDF classic, 64-bit offset) ------------------ */
+
+/* Write the solver's arrays as a QES-Winds workspace file: the reference's
+ * WINDSOutputWorkspace layout (WINDSOutputWorkspace.cpp:40-150) for u, v, w,
+ * u0, v0, w0, icellflag, e..n, dz_array and the coordinates, one time
+ * record.  The reference writes NetCDF-4; the netCDF library reads this
+ * classic format the same way. */
+int qesmc_workspace_write(const char *path, const qesmc_grid *grid, const qesmc_coeffs *coeffs,
+                          const qesmc_fields *fields);
+
+/* Staggered sizes of a workspace file (x_face, y_face, z_face dimensions,
+ * as Domain(const std::string&) reads them, Domain.cpp:59-66). */
+int qesmc_workspace_dims(const char *path, int *nx, int *ny, int *nz);
+
+/* Read a workspace file (NetCDF classic or 64-bit offset; a NetCDF-4 file
+ * converts with `nccopy -k 64-bit-offset`) into caller buffers of the given
+ * staggered sizes, which must match the file; any pointer may be NULL.
+ * dx = x[1] - x[0], dy = y[1] - y[0], dz_array from the file or uniform
+ * z[1] - z[0] (Domain.cpp:59-109); e..n and icellflag as the coefficient
+ * read of WINDSGeneralData.cpp:880-911; record 0 of the time dimension;
+ * numeric types other than float / int are converted. */
+int qesmc_workspace_read(const char *path, int nx, int ny, int nz, float *dx, float *dy, float *dz_array,
+                         float *e, float *f, float *g, float *h, float *m, float *n, int *icellflag, float *u0,
+                         float *v0, float *w0, float *u, float *v, float *w);
 
 /* ---- learned warm start: 3-D U-Net predicting lambda0 --------------------
  *

@@ -203,6 +203,7 @@ const char *qesmc_status_string(int s)
   case QESMC_ERR_NONFINITE: return "lambda became non-finite";
   case QESMC_ERR_STATE: return "inputs not set";
   case QESMC_ERR_UNSUPPORTED: return "unsupported option";
+  case QESMC_ERR_IO: return "file missing, unwritable or not a NetCDF classic file";
   default: return "unknown status";
   }
 }

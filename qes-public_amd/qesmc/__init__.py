@@ -27,6 +27,7 @@ ERR_ZERO_DIAG = -5
 ERR_NONFINITE = -6
 ERR_STATE = -7
 ERR_UNSUPPORTED = -8
+ERR_IO = -9
 
 PATH_AUTO = 0
 PATH_PLAIN = 1
@@ -434,6 +435,66 @@ def _city_lib():
         L.qesmc_velocity_gradient.restype = C.c_int
         L._city_sigs_set = True
     return L
+
+
+EXPORTS = EXPORTS + ("qesmc_workspace_write", "qesmc_workspace_dims", "qesmc_workspace_read")
+
+
+def _ws_lib():
+    L = lib()
+    if not getattr(L, "_ws_sigs_set", False):
+        L.qesmc_workspace_write.argtypes = [C.c_char_p, C.POINTER(Grid), C.POINTER(Coeffs), C.POINTER(Fields)]
+        L.qesmc_workspace_write.restype = C.c_int
+        L.qesmc_workspace_dims.argtypes = [C.c_char_p] + [C.POINTER(C.c_int)] * 3
+        L.qesmc_workspace_dims.restype = C.c_int
+        L.qesmc_workspace_read.argtypes = ([C.c_char_p, C.c_int, C.c_int, C.c_int, _fp, _fp, _fp] + [_fp] * 6 + [_ip]
+                                           + [_fp] * 6)
+        L.qesmc_workspace_read.restype = C.c_int
+        L._ws_sigs_set = True
+    return L
+
+
+def workspace_write(path, nx, ny, nz, dx, dy, dz_array, coeffs, icellflag, u0, v0, w0, u, v, w):
+    """Write a QES-Winds workspace file, NetCDF classic (qesmc_workspace_write).
+    coeffs = (e, f, g, h, m, n) cell arrays; u0..w face arrays."""
+    dz_array = np.ascontiguousarray(dz_array, np.float32)
+    co = [np.ascontiguousarray(a, np.float32) for a in coeffs]
+    flag = np.ascontiguousarray(icellflag, np.int32)
+    fl = [np.ascontiguousarray(a, np.float32) for a in (u0, v0, w0, u, v, w)]
+    g = Grid(int(nx), int(ny), int(nz), float(dx), float(dy), _f(dz_array))
+    cs = Coeffs(*[_f(a) for a in co], _i(flag))
+    fs = Fields(*[_f(a) for a in fl])
+    _check(_ws_lib().qesmc_workspace_write(os.fsencode(path), C.byref(g), C.byref(cs), C.byref(fs)),
+           "qesmc_workspace_write")
+
+
+def workspace_dims(path):
+    """Staggered (nx, ny, nz) of a workspace file (qesmc_workspace_dims)."""
+    d = [C.c_int() for _ in range(3)]
+    _check(_ws_lib().qesmc_workspace_dims(os.fsencode(path), *[C.byref(v) for v in d]), "qesmc_workspace_dims")
+    return tuple(v.value for v in d)
+
+
+def workspace_read(path, velocity=True):
+    """Arrays of a workspace file (qesmc_workspace_read): a dict with nx, ny,
+    nz, dx, dy, dz_array, e..n, icellflag, u0, v0, w0 and, with velocity,
+    u, v, w."""
+    nx, ny, nz = workspace_dims(path)
+    ncell, nface = (nx - 1) * (ny - 1) * (nz - 1), nx * ny * nz
+    dx, dy = C.c_float(), C.c_float()
+    out = {"nx": nx, "ny": ny, "nz": nz, "dz_array": np.empty(nz - 1, np.float32)}
+    for k in ("e", "f", "g", "h", "m", "n"):
+        out[k] = np.empty(ncell, np.float32)
+    out["icellflag"] = np.empty(ncell, np.int32)
+    for k in ("u0", "v0", "w0") + (("u", "v", "w") if velocity else ()):
+        out[k] = np.empty(nface, np.float32)
+    vel = [_f(out[k]) if k in out else None for k in ("u", "v", "w")]
+    _check(_ws_lib().qesmc_workspace_read(os.fsencode(path), nx, ny, nz, C.byref(dx), C.byref(dy),
+                                          _f(out["dz_array"]), *[_f(out[k]) for k in ("e", "f", "g", "h", "m", "n")],
+                                          _i(out["icellflag"]), _f(out["u0"]), _f(out["v0"]), _f(out["w0"]), *vel),
+           "qesmc_workspace_read")
+    out["dx"], out["dy"] = dx.value, dy.value
+    return out
 
 
 def lambda_combine(ctx: "Context", srcs, coefs):
