@@ -226,6 +226,29 @@ def test_city_400x400x120_reference_schedule(gpu_ctx, oracle, qesmc):
         assert_same(w, ow, "w")
 
 
+def test_city_1000x1000x150_bit_exact(oracle, qesmc):
+    """The largest workload: 1000 x 1000 x 150 cells (151 M), dx = 1 m,
+    geometric dz, 3000 lognormal buildings, log law from 270 deg; 12
+    iterations on every path, bit-exact against the oracle (whose OpenMP
+    loop runs them on the host cores in seconds)."""
+    dz = qesmc.dz_geometric(150, 1.0, 1.02)
+    blds = qesmc.random_buildings(555, 1000, 1000, 1.0, 3000, 6.0, 16.0, 20.0, 0.6, lognormal=True, hcap=120.0,
+                                  margin=2)
+    u_ref = 0.5 / 0.4 * math.log((10.0 + 0.5) / 0.5)
+    p = oracle.build_city(1000, 1000, 1.0, dz, blds, (1, u_ref, 10.0, 0.5, 270.0))
+    assert p.ncell == 1000 * 1000 * 151
+    ou, ov, ow, olam, oit, ome = oracle.solve_rb_f32(p, itermax=12)
+    with qesmc.Context(0) as ctx:
+        for path in (SPLIT, FUSED, PLAIN):
+            u, v, w, lam, res = ctx.solve(p, path=path, itermax=12)
+            assert res["iterations"] == oit == 12 and res["path"] == path
+            assert np.float32(res["final_max_dlambda"]) == np.float32(ome)
+            assert_same(lam, olam, f"lambda path {path}")
+            assert_same(u, ou, "u")
+            assert_same(v, ov, "v")
+            assert_same(w, ow, "w")
+
+
 @pytest.mark.parametrize("tile", [(1, 1, 1), (3, 1, 2), (8, 2, 8), (16, 4, 4), (5, 4, 16), (64, 2, 1)])
 def test_tile_parameters_never_change_results(gpu_ctx, oracle, qesmc, tile):
     """(tile_k, tile_v, wg_per_cu) reshape the work decomposition only."""
