@@ -48,7 +48,7 @@ __global__ __launch_bounds__(256) void k_sor_split(Dims d, int colour, int kb, i
                                                    const float *__restrict__ cm,
                                                    const float *__restrict__ cn, float omega,
                                                    float one_m_omega, float tol, unsigned *err, int t,
-                                                   int itermax, unsigned *status)
+                                                   int itermax, unsigned *status, int reverse)
 {
   const int lane = threadIdx.x & 63;
   // uniform across the wave: tile indices then live in scalar registers
@@ -68,8 +68,13 @@ __global__ __launch_bounds__(256) void k_sor_split(Dims d, int colour, int kb, i
 
   float dl = 0.f;
   for (long long wt = tbeg + (long long)local * 4 + wave; wt < tend; wt += (long long)nblk * 4) {
-    const int kc = (int)(wt / tiles_per_plane);
-    const int pt = (int)(wt - (long long)kc * tiles_per_plane);
+    // with `reverse`, the black pass walks each XCD's tile range backwards:
+    // each pass then starts on the planes the previous pass touched last,
+    // still in that XCD's L2 and in the Infinity Cache (reversing the whole
+    // range instead would hand each XCD another XCD's slab)
+    const long long wr = (colour && reverse) ? tbeg + tend - 1 - wt : wt;
+    const int kc = (int)(wr / tiles_per_plane);
+    const int pt = (int)(wr - (long long)kc * tiles_per_plane);
     const int p = pt * (64 * VQ) + lane * VQ;
     const int jr = p / d.HP;
     const int j = jr + 1;
@@ -142,6 +147,11 @@ hipError_t qesmc_launch_split_iteration(qesmc_ctx *x, bool masked, int t, int it
   if (blocks > cap) blocks = cap;
   if (blocks < kXcds) blocks = kXcds;  // every XCD's tile range needs a work-group
   const float omo = 1.0f - omega;
+  // Reverse the black pass only when the pass's arrays (lambda and R of both
+  // colours and the masks) exceed MI355X's 256 MB Infinity Cache: measured
+  // 2.7 % faster per iteration on 1000 x 1000 x 150, 0.8 % slower on
+  // 400 x 400 x 120, whose arrays stay cache-resident either way.
+  const int reverse = (double)x->nsplit * 2.0 * (4.0 + 4.0 + 1.0) > 256.0 * (1 << 20) ? 1 : 0;
   for (int colour = 0; colour < 2; ++colour) {
     float *own = x->d_lc[colour], *oth = x->d_lc[colour ^ 1];
     const float *const *cc = x->d_cc[colour];
@@ -149,7 +159,7 @@ hipError_t qesmc_launch_split_iteration(qesmc_ctx *x, bool masked, int t, int it
 #define QESMC_SPLIT_LAUNCH(M, V)                                                                          \
   hipLaunchKernelGGL((k_sor_split<M, V>), dim3((unsigned)blocks), dim3(256), 0, x->stream, d, colour, kb, \
                      tiles_per_plane, ntiles, own, oth, x->d_Rc[colour], mk, x->d_canon, cc[0], cc[1],    \
-                     cc[2], cc[3], cc[4], cc[5], omega, omo, tol, x->d_err, t, itermax, x->d_status)
+                     cc[2], cc[3], cc[4], cc[5], omega, omo, tol, x->d_err, t, itermax, x->d_status, reverse)
     if (masked) {
       if (vq == 4) QESMC_SPLIT_LAUNCH(true, 4);
       else if (vq == 2) QESMC_SPLIT_LAUNCH(true, 2);
