@@ -399,6 +399,11 @@ int run_prepare(qesmc_ctx *x, const qesmc_params *pp, RunState &st)
   // lane, 5 for one (fewer, longer-lived work-groups beat full occupancy on
   // every measured size but the HBM-bound 1000x1000x150, where they lose 1 %)
   x->wg_per_cu = p.wg_per_cu > 0 ? p.wg_per_cu : x->wg_per_cu_auto > 0 ? x->wg_per_cu_auto : x->tile_v >= 2 ? 4 : 5;
+  if (st.path == QESMC_PATH_PLAIN) {
+    // the classic layout's own default: 8 planes per work-group (its
+    // measured best; the split path's choice above does not apply to it)
+    x->kplanes = p.tile_k > 0 ? p.tile_k : 8;
+  }
   if (st.path == QESMC_PATH_FUSED) {
     // one work-group per 124 x 8 x tile_k tile (tile_v and wg_per_cu unused);
     // 12 planes measured best on 400x400x120 and 1000x1000x150
