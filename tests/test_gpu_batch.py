@@ -92,3 +92,31 @@ def test_autotune_keeps_results(qesmc, oracle):
         lam = c.get_lambda()
     olam = oracle.solve_rb_f32(p, itermax=60)[3]
     assert np.array_equal(lam, olam)
+
+
+def test_batch_larger_than_the_infinity_cache(qesmc, oracle):
+    """Two 400 x 400 x 120 cities in one batch (≈ 350 MB of split arrays,
+    more than the 256 MB Infinity Cache): per-city results bitwise equal to
+    the oracle's."""
+    import math
+    dz = qesmc.dz_geometric(120, 1.0, 1.02)
+    u_ref = 0.5 / 0.4 * math.log((10.0 + 0.5) / 0.5)
+    probs = [oracle.build_city(400, 400, 2.0, dz,
+                               qesmc.random_buildings(2024 + q, 400, 400, 2.0, 200, 10.0, 40.0, 10.0, 60.0, margin=2),
+                               (1, u_ref, 10.0, 0.5, 217.0 + 40.0 * q)) for q in range(2)]
+    ctxs = [qesmc.Context(0) for _ in probs]
+    try:
+        for c, p in zip(ctxs, probs):
+            c.set_geometry(p)
+            c.set_inflow(p.u0, p.v0, p.w0)
+            c.set_lambda(None)
+        res, _ = qesmc.run_batch(ctxs, itermax=7, tol=1e-9, check_every=4, use_graph=True)
+        for c, p, r in zip(ctxs, probs, res):
+            ou, ov, ow, olam, oit, ome = oracle.solve_rb_f32(p, itermax=7)
+            assert r["iterations"] == oit == 7
+            assert np.array_equal(c.get_lambda(), olam)
+            u, v, w = c.get_velocity()
+            assert np.array_equal(u, ou) and np.array_equal(v, ov) and np.array_equal(w, ow)
+    finally:
+        for c in ctxs:
+            c.close()
