@@ -85,7 +85,7 @@ __global__ __launch_bounds__(256) void k_sor_batch(const BatchItem *__restrict__
   QESMC_GLOBAL float *own = nullptr;
   const QESMC_GLOBAL float *oth = nullptr, *Rc = nullptr;
   const QESMC_GLOBAL unsigned char *mk = nullptr;
-  const QESMC_GLOBAL unsigned *canon = nullptr;
+  const QESMC_CONSTANT unsigned *canon = nullptr;
   unsigned *status = nullptr;
   float dl = 0.f;
   for (long long wt = tbeg + (long long)local * 4 + wave; wt < tend; wt += step) {
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(256) void k_sor_batch(const BatchItem *__restrict__
       oth = as_global(it.lc[colour ^ 1]);
       Rc = as_global(it.Rc[colour]);
       mk = as_global(it.mask[colour]);
-      canon = as_global(it.canon);
+      canon = as_constant(it.canon);
       status = it.status;
       masked = ALL_MASKED || it.masked != 0;
     }
@@ -142,13 +142,12 @@ __global__ __launch_bounds__(256) void k_sor_batch(const BatchItem *__restrict__
     float *o_ = (float *)own;
     const float *oth_ = (const float *)oth, *Rc_ = (const float *)Rc;
     const unsigned char *mk_ = (const unsigned char *)mk;
-    const unsigned *canon_ = (const unsigned *)canon;
     if (masked) {
-      dl = sor_tile<true, VQ>(d, colour, t, j, q, k0, k1, o_, oth_, Rc_, mk_, canon_, nullptr, nullptr, nullptr,
+      dl = sor_tile<true, VQ>(d, colour, t, j, q, k0, k1, o_, oth_, Rc_, mk_, canon, nullptr, nullptr, nullptr,
                               nullptr, nullptr, nullptr, omega, one_m_omega, status, dl);
     } else if (!ALL_MASKED) {
       const float *const *cc = items[cur].cc[colour];
-      dl = sor_tile<false, VQ>(d, colour, t, j, q, k0, k1, o_, oth_, Rc_, mk_, canon_,
+      dl = sor_tile<false, VQ>(d, colour, t, j, q, k0, k1, o_, oth_, Rc_, mk_, canon,
                               (const float *)as_global(cc[0]), (const float *)as_global(cc[1]),
                               (const float *)as_global(cc[2]), (const float *)as_global(cc[3]),
                               (const float *)as_global(cc[4]), (const float *)as_global(cc[5]), omega, one_m_omega,
@@ -236,7 +235,11 @@ int qesmc_run_batch(qesmc_ctx **ctxs, int n, const qesmc_params *params, qesmc_r
   const int vq = params->tile_v == 0 ? 2 : params->tile_v;
   if (vq != 1 && vq != 2) return QESMC_ERR_UNSUPPORTED;
   qesmc_params pv = *params;
-  pv.tile_v = vq;  // each instance picks its tile_k for this tile_v
+  pv.tile_v = vq;
+  // one plane chunk for every member unless the caller sets one: chunks
+  // autotuned for each member's single-problem solves made a batch of 64
+  // uneven cities 13-28 % slower than a uniform chunk of 4 planes
+  if (pv.tile_k <= 0) pv.tile_k = 4;
   params = &pv;
   qesmc_ctx *lead = ctxs[0];
   BCHK(hipSetDevice(lead->device));
@@ -318,10 +321,11 @@ int qesmc_run_batch(qesmc_ctx **ctxs, int n, const qesmc_params *params, qesmc_r
   bool all_masked = true;
   for (int i = 0; i < n; ++i) all_masked = all_masked && st[i].masked;
   // persistent grid: never more work-groups than can be resident at once;
-  // untuned, 4 per CU for two half-cells per lane and 5 for one, as for the
-  // single-problem kernel (8 was measured 10-15 % slower on a batch of two
-  // 400 x 400 x 120 cities)
-  int wg = p.wg_per_cu > 0 ? p.wg_per_cu : lead->wg_per_cu_auto > 0 ? lead->wg_per_cu_auto : vq == 2 ? 4 : 5;
+  // unless the caller sets it, 5 per CU (a batch of 64 uneven cities: 8-10 %
+  // faster than 4, 1 % faster than 6; 8 was 10-15 % slower on a batch of two
+  // 400 x 400 x 120 cities).  Members' autotuned values tune the
+  // single-problem kernel and are not used here.
+  int wg = p.wg_per_cu > 0 ? p.wg_per_cu : 5;
   {
     int fit = 0;
     const hipError_t oe =

@@ -20,6 +20,17 @@ __device__ inline QESMC_GLOBAL T *as_global(T *p)
   return (QESMC_GLOBAL T *)p;
 }
 
+// Read-only data that a kernel never writes, addressed through a pointer read
+// from memory: in the constant address space, wave-uniform loads from it go
+// through the scalar cache (s_load), as they do for a const __restrict__
+// kernel argument; through a global pointer they would be vector loads.
+#define QESMC_CONSTANT __attribute__((address_space(4)))
+template <typename T>
+__device__ inline const QESMC_CONSTANT T *as_constant(const T *p)
+{
+  return (const QESMC_CONSTANT T *)p;
+}
+
 // Status bits raised by kernels into ctx->d_status.
 enum : unsigned {
   ST_NONFINITE = 1u,      // a relaxed lambda was NaN or Inf

@@ -80,11 +80,14 @@ __device__ __forceinline__ float relax(float ec, float fc, float gc, float hc, f
          + one_m_omega * o;
 }
 
-template <bool MASKED, int VQ>
+// CanonP: `const unsigned *` (a kernel argument) or a constant-address-space
+// pointer (read from a table); either way the six wall values of a layer are
+// wave-uniform scalar loads.
+template <bool MASKED, int VQ, typename CanonP>
 __device__ __forceinline__ float sor_tile(const Dims &d, int colour, int t, int j, int q, int k0, int k1,
                                           float *__restrict__ own, const float *__restrict__ oth,
                                           const float *__restrict__ Rc, const unsigned char *__restrict__ mask,
-                                          const unsigned *__restrict__ canon, const float *__restrict__ ce,
+                                          CanonP __restrict__ canon, const float *__restrict__ ce,
                                           const float *__restrict__ cf, const float *__restrict__ cg,
                                           const float *__restrict__ ch, const float *__restrict__ cm,
                                           const float *__restrict__ cn, float omega, float one_m_omega,
