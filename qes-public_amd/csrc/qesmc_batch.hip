@@ -392,6 +392,7 @@ int qesmc_run_batch(qesmc_ctx **ctxs, int n, const qesmc_params *params, qesmc_r
   if ((e = hipEventRecord(e1, bs)) != hipSuccess) return fail(e);
   for (int i = 0; i < n; ++i) {
     const int r = run_finish(ctxs[i], st[i], any_runs ? launched : 0, results ? &results[i] : nullptr);
+    if (results) results[i].wg_per_cu = wg;  // the batch's launch, not the context's own default
     if (r != QESMC_OK && rc == QESMC_OK) rc = r;
   }
   if (ms_loop) {

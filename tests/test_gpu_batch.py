@@ -94,6 +94,33 @@ def test_autotune_keeps_results(qesmc, oracle):
     assert np.array_equal(lam, olam)
 
 
+def test_batch_reports_its_tiling(qesmc, oracle):
+    """One tiling serves the whole batch (defaults tile_k 4, tile_v 2, 5
+    work-groups per CU, or the caller's); a member's autotuned tiling is left
+    to its single solves."""
+    probs = [city(oracle, qesmc, 410 + q, *s) for q, s in enumerate(SHAPES[:2])]
+    ctxs = [qesmc.Context(0) for _ in probs]
+    try:
+        for c, p in zip(ctxs, probs):
+            c.set_geometry(p)
+            c.set_inflow(p.u0, p.v0, p.w0)
+        best = qesmc.autotune(ctxs[0], iterations=10)
+        res, _ = qesmc.run_batch(ctxs, itermax=20, tol=1e-9, use_graph=False)
+        assert [(r["tile_k"], r["tile_v"], r["wg_per_cu"]) for r in res] == [(4, 2, 5)] * 2
+        for c in ctxs:
+            c.set_lambda(None)
+        res, _ = qesmc.run_batch(ctxs, itermax=20, tol=1e-9, use_graph=False, tile_k=2, tile_v=1, wg_per_cu=3)
+        assert [(r["tile_k"], r["tile_v"], r["wg_per_cu"]) for r in res] == [(2, 1, 3)] * 2
+        lam_b = ctxs[0].get_lambda()
+        ctxs[0].set_lambda(None)
+        r = ctxs[0].run(itermax=20, tol=1e-9)
+        assert (r["tile_k"], r["tile_v"], r["wg_per_cu"]) == (best.tile_k, best.tile_v, best.wg_per_cu)
+        assert np.array_equal(ctxs[0].get_lambda(), lam_b)
+    finally:
+        for c in ctxs:
+            c.close()
+
+
 def test_batch_larger_than_the_infinity_cache(qesmc, oracle):
     """Two 400 x 400 x 120 cities in one batch (≈ 350 MB of split arrays,
     more than the 256 MB Infinity Cache): per-city results bitwise equal to
