@@ -288,6 +288,7 @@ def batch_leg(n=64, itermax=500, reps=2):
                     c.set_lambda(None)
                 res, ms_loop = Q.run_batch(ctxs, itermax=itermax, tol=1e-9, check_every=50, use_graph=g)
                 ms.append(ms_loop)
+                batch_tiling = (res[0]["tile_k"], res[0]["tile_v"], res[0]["wg_per_cu"])
             best["graph" if g else "eager"] = min(ms)
         seq = []
         for _ in range(reps):
@@ -309,8 +310,8 @@ def batch_leg(n=64, itermax=500, reps=2):
         "glups_batched": round(lups / best["graph"] / 1e6, 2), "glups_sequential": round(lups / seq_ms / 1e6, 2),
         "speedup_batched_vs_sequential": round(seq_ms / best["graph"], 3),
         "device_city_build_s": round(build_s, 3), "autotune_s": round(tune_s, 3),
-        "tiling": "batched: one tiling for all cities (tile_k 4, tile_v 2, 5 work-groups per CU); "
-                  "sequential: each city's autotuned tiling",
+        "tiling": "batched: one tiling for all cities (tile_k %d, tile_v %d, %d work-groups per CU); "
+                  "sequential: each city's autotuned tiling" % batch_tiling,
     }
 
 

@@ -169,9 +169,11 @@ int qesmc_divergence_linf(qesmc_ctx *ctx, int physical, double *out);
  * iterations are replayed as a captured HIP graph of check_every iterations.
  * Per problem the results equal qesmc_run's; results[i] (optional) receives
  * them (their timings span the whole batch).  ms_loop (optional): device
- * time of the batched iterations.  Split path only; tile_v 1 or 2 (0: 2),
- * tile_k (0: 4) and wg_per_cu (0: 5) apply to the whole batch; tilings
- * autotuned per context are not used. */
+ * time of the batched iterations.  Split path only.  One tiling serves the
+ * whole batch: tile_v 1 or 2 and tile_k as given, else the most work per
+ * wave tile (at most 4 planes x 2 half-cells) that leaves >= 6 wave tiles
+ * per CU over all members; wg_per_cu as given, else 5.  Tilings autotuned
+ * per context are not used. */
 int qesmc_run_batch(qesmc_ctx **ctxs, int n, const qesmc_params *params, qesmc_result *results,
                     int use_graph, double *ms_loop);
 

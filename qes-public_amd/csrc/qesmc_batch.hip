@@ -231,15 +231,45 @@ int qesmc_run_batch(qesmc_ctx **ctxs, int n, const qesmc_params *params, qesmc_r
       if (ctxs[j] == ctxs[i]) return QESMC_ERR_ARG;
   }
   if (params->path == QESMC_PATH_PLAIN || params->path == QESMC_PATH_FUSED) return QESMC_ERR_UNSUPPORTED;
-  // half-cells per lane, one value for the whole batch: 1 or 2 (default 2)
-  const int vq = params->tile_v == 0 ? 2 : params->tile_v;
-  if (vq != 1 && vq != 2) return QESMC_ERR_UNSUPPORTED;
+  // One tiling for the whole batch (half-cells per lane 1 or 2, planes per
+  // wave tile) unless the caller sets it: the most work per wave tile, at
+  // most 4 planes x 2 half-cells, that leaves >= 6 wave tiles per CU over all
+  // members, the single-problem rule applied to the batch.  On a batch of 64
+  // uneven cities that is 4 x 2, 13-28 % faster than taking each member's
+  // tiling autotuned for its single solves.
+  int kb = params->tile_k > 0 ? params->tile_k : 0;
+  int vq = params->tile_v;
+  if (vq != 0 && vq != 1 && vq != 2) return QESMC_ERR_UNSUPPORTED;
+  if (kb == 0 || vq == 0) {
+    static const int order[][2] = {{4, 2}, {2, 2}, {4, 1}, {2, 1}, {1, 2}, {1, 1}};
+    int bk = 0, bv = 0;
+    long long most = -1;
+    for (const auto &o : order) {
+      if ((kb && o[0] != kb) || (vq && o[1] != vq)) continue;
+      long long tiles = 0;
+      for (int i = 0; i < n; ++i) {
+        const Dims &d = ctxs[i]->d;
+        if (d.NY < 3 || d.NZ < 3) continue;
+        const long long flat = (long long)(d.NY - 2) * d.HP;
+        tiles += ((flat + 64LL * o[1] - 1) / (64LL * o[1])) * ((d.NZ - 2 + o[0] - 1) / o[0]);
+      }
+      if (tiles >= 6LL * ctxs[0]->n_cu) {
+        bk = o[0];
+        bv = o[1];
+        break;
+      }
+      if (tiles > most) {
+        most = tiles;
+        bk = o[0];
+        bv = o[1];
+      }
+    }
+    if (!kb) kb = bk ? bk : 4;
+    if (!vq) vq = bv ? bv : 2;
+  }
   qesmc_params pv = *params;
   pv.tile_v = vq;
-  // one plane chunk for every member unless the caller sets one: chunks
-  // autotuned for each member's single-problem solves made a batch of 64
-  // uneven cities 13-28 % slower than a uniform chunk of 4 planes
-  if (pv.tile_k <= 0) pv.tile_k = 4;
+  pv.tile_k = kb;
   params = &pv;
   qesmc_ctx *lead = ctxs[0];
   BCHK(hipSetDevice(lead->device));
