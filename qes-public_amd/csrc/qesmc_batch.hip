@@ -350,12 +350,17 @@ int qesmc_run_batch(qesmc_ctx **ctxs, int n, const qesmc_params *params, qesmc_r
 
   bool all_masked = true;
   for (int i = 0; i < n; ++i) all_masked = all_masked && st[i].masked;
-  // persistent grid: never more work-groups than can be resident at once;
-  // unless the caller sets it, 5 per CU (a batch of 64 uneven cities: 8-10 %
-  // faster than 4, 1 % faster than 6; 8 was 10-15 % slower on a batch of two
-  // 400 x 400 x 120 cities).  Members' autotuned values tune the
-  // single-problem kernel and are not used here.
-  int wg = p.wg_per_cu > 0 ? p.wg_per_cu : 5;
+  // persistent grid: never more work-groups than can be resident at once.
+  // Unless the caller sets it: 5 per CU when the members' split arrays
+  // (lambda, R and mask of both colours) exceed the 256 MB Infinity Cache and
+  // each pass streams from HBM (a batch of 64 uneven cities, 1.2 GB: 8-10 %
+  // faster than 4, 1 % faster than 6), else the single-problem kernel's
+  // choice, 4 for two half-cells per lane and 5 for one (one 256 x 256 x 96
+  // city: 5 is 10 % slower than 4 at two).  Members' autotuned values tune
+  // the single-problem kernel and are not used here.
+  double split_bytes = 0.0;
+  for (int i = 0; i < n; ++i) split_bytes += (double)ctxs[i]->nsplit * 2.0 * (4.0 + 4.0 + 1.0);
+  int wg = p.wg_per_cu > 0 ? p.wg_per_cu : split_bytes > 256.0 * (1 << 20) ? 5 : vq == 2 ? 4 : 5;
   {
     int fit = 0;
     const hipError_t oe =
