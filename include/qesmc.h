@@ -172,9 +172,9 @@ int qesmc_divergence_linf(qesmc_ctx *ctx, int physical, double *out);
  * time of the batched iterations.  Split path only.  One tiling serves the
  * whole batch: tile_v 1 or 2 and tile_k as given, else the most work per
  * wave tile (at most 4 planes x 2 half-cells) that leaves >= 6 wave tiles
- * per CU over all members; wg_per_cu as given, else 5 when the members'
- * split arrays exceed 256 MB (the Infinity Cache) and qesmc_run's untuned
- * choice below that.  Tilings autotuned per context are not used. */
+ * per CU over all members; wg_per_cu as given, else 5 from 48 wave tiles
+ * per CU up and qesmc_run's untuned choice below that.  Tilings autotuned
+ * per context are not used. */
 int qesmc_run_batch(qesmc_ctx **ctxs, int n, const qesmc_params *params, qesmc_result *results,
                     int use_graph, double *ms_loop);
 

@@ -98,8 +98,8 @@ def test_autotune_keeps_results(qesmc, oracle):
 def test_batch_reports_its_tiling(qesmc, oracle):
     """One tiling serves the whole batch: unless the caller sets it, the most
     work per wave tile (4 planes x 2 half-cells down to 1 x 1) that leaves at
-    least 6 wave tiles per CU over all members, and 4 work-groups per CU while
-    the members' split arrays fit the 256 MB Infinity Cache (5 beyond).  A
+    least 6 wave tiles per CU over all members, and 5 work-groups per CU from
+    48 wave tiles per CU up (4 below, at two half-cells per lane).  A
     128 x 128 x 64 city has 127 rows of 68 half-cells and 64 updated planes:
     1088 wave tiles of 4 x 2, 2176 of 2 x 2; MI355X has 256 CUs (1536 tiles
     needed).  A member's autotuned tiling is left to its single solves."""
@@ -145,7 +145,7 @@ def test_batch_larger_than_the_infinity_cache(qesmc, oracle):
             c.set_inflow(p.u0, p.v0, p.w0)
             c.set_lambda(None)
         res, _ = qesmc.run_batch(ctxs, itermax=7, tol=1e-9, check_every=4, use_graph=True)
-        assert [(r["tile_k"], r["tile_v"], r["wg_per_cu"]) for r in res] == [(4, 2, 5)] * 2  # beyond the cache: 5
+        assert [(r["tile_k"], r["tile_v"], r["wg_per_cu"]) for r in res] == [(4, 2, 5)] * 2  # 149 tiles per CU
         for c, p, r in zip(ctxs, probs, res):
             ou, ov, ow, olam, oit, ome = oracle.solve_rb_f32(p, itermax=7)
             assert r["iterations"] == oit == 7
