@@ -18,10 +18,13 @@
 // file has it, else uniform z[1] - z[0].  Coefficients and flags are read as
 // WINDSGeneralData does with readCoefficientsFlag (WINDSGeneralData.cpp:
 // 880-911), one record of (t, z, y, x) arrays, here the whole arrays.
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "qesmc.h"
@@ -236,12 +239,17 @@ struct NcFileIn {
     return nullptr;
   }
   // number of values of one record (or of the whole fixed-size variable)
+  // (saturates at 2^48, far above any array this library accepts, so that a
+  // malformed header cannot wrap the product onto an expected size)
   uint64_t count(const RVar &v) const
   {
+    const uint64_t cap = (uint64_t)1 << 48;
     uint64_t n = 1;
     for (size_t a = 0; a < v.dims.size(); ++a) {
       if (a == 0 && v.record) continue;
-      n *= dims[v.dims[a]].len;
+      const uint64_t len = dims[v.dims[a]].len;
+      if (len != 0 && n > cap / len) return cap;
+      n *= len;
     }
     return n;
   }
@@ -286,15 +294,22 @@ int open_header(FILE *f, NcFileIn &nc)
         const uint32_t nd = r.u32();
         for (uint32_t q = 0; q < nd && r.ok; ++q) {
           const uint32_t id = r.u32();
-          if (id >= nc.dims.size()) r.ok = false;
+          if (!r.ok || id >= nc.dims.size()) {
+            r.ok = false;  // an invalid dimension id ends the parse before it is used
+            break;
+          }
           v.dims.push_back(id);
         }
+        if (!r.ok) break;
         r.skip_atts();
         v.type = (int)r.u32();
         v.vsize = r.u32();
         v.begin = nc.version == 2 ? r.u64() : r.u32();
+        if (!r.ok || !type_size(v.type)) {
+          r.ok = false;
+          break;
+        }
         v.record = !v.dims.empty() && nc.dims[v.dims[0]].len == 0;
-        if (!type_size(v.type)) r.ok = false;
         nc.vars.push_back(v);
       }
     } else if (tag != 0 || n != 0) {
@@ -318,6 +333,17 @@ int open_header(FILE *f, NcFileIn &nc)
       if (v.record) rs = nc.count(v) * type_size(v.type);
   nc.recsize = rs;
   return QESMC_OK;
+}
+
+// whether a value converts to T without undefined behaviour: a float array
+// takes NaN, infinities and every finite double within float's range; an
+// integer array only non-NaN values within its range
+template <typename T>
+bool in_range(double val)
+{
+  if (std::is_floating_point<T>::value)
+    return !(std::isfinite(val) && std::fabs(val) > (double)std::numeric_limits<T>::max());
+  return val == val && val >= (double)std::numeric_limits<T>::min() && val <= (double)std::numeric_limits<T>::max();
 }
 
 template <typename T>
@@ -355,6 +381,7 @@ int read_as(FILE *f, const NcFileIn &nc, const RVar &v, uint64_t expect, T *out)
       }
       default: return QESMC_ERR_IO;
       }
+      if (!in_range<T>(val)) return QESMC_ERR_IO;  // NaN or out of range for an integer array
       out[i + q] = (T)val;
     }
   }
@@ -365,6 +392,7 @@ int file_dims(const NcFileIn &nc, int *nx, int *ny, int *nz)
 {
   const RDim *xf = nc.dim("x_face"), *yf = nc.dim("y_face"), *zf = nc.dim("z_face");
   if (!xf || !yf || !zf) return QESMC_ERR_IO;
+  if (xf->len > 0x7fffffffu || yf->len > 0x7fffffffu || zf->len > 0x7fffffffu) return QESMC_ERR_IO;
   *nx = (int)xf->len;
   *ny = (int)yf->len;
   *nz = (int)zf->len;

@@ -68,9 +68,11 @@ def test_round_trip_through_the_library(qesmc, oracle, tmp_path):
         assert np.array_equal(r[name], arr), name
 
 
-def scipy_workspace(path, p, version, with_dz=True, dz_double=False, uniform_dz=None):
+def scipy_workspace(path, p, version, with_dz=True, dz_double=False, uniform_dz=None, flag_type="i", flag_fill=None):
     """A workspace file written by scipy: other variable order, extra
-    variables and attributes, two time records, optional dz_array."""
+    variables and attributes, two time records, optional dz_array;
+    icellflag of netCDF type flag_type, optionally with flag_fill stored in
+    one cell."""
     with netcdf_file(path, "w", version=version) as nc:
         nc.title = "written by scipy"
         nc.createDimension("t", None)
@@ -97,8 +99,11 @@ def scipy_workspace(path, p, version, with_dz=True, dz_double=False, uniform_dz=
             d[:] = p.dz_array
         shape_c = (2, p.nz - 1, p.ny - 1, p.nx - 1)
         shape_f = (2, p.nz, p.ny, p.nx)
-        flag = nc.createVariable("icellflag", "i", ("t", "z", "y", "x"))
-        flag[:] = np.stack([p.icellflag.reshape(shape_c[1:]), np.zeros(shape_c[1:], np.int32)])
+        flag = nc.createVariable("icellflag", flag_type, ("t", "z", "y", "x"))
+        fl0 = p.icellflag.reshape(shape_c[1:]).astype(flag.data.dtype)
+        if flag_fill is not None:
+            fl0.flat[fl0.size // 2] = flag_fill
+        flag[:] = np.stack([fl0, np.zeros(shape_c[1:], fl0.dtype)])
         for name, arr in zip("nmhgfe", p.coeffs[::-1]):
             var = nc.createVariable(name, "f", ("t", "z", "y", "x"))
             var[:] = np.stack([arr.reshape(shape_c[1:]), np.ones(shape_c[1:], np.float32)])
@@ -159,3 +164,19 @@ def test_errors(qesmc, oracle, tmp_path):
     with pytest.raises(qesmc.QesmcError) as ei:
         qesmc.workspace_read(nov, velocity=True)
     assert ei.value.code == qesmc.ERR_ARG
+
+
+@pytest.mark.parametrize("flag_type,fill", [("f", float("nan")), ("d", 1e20), ("d", -3e9), ("f", float("inf"))])
+def test_icellflag_values_beyond_int_are_refused(qesmc, oracle, tmp_path, flag_type, fill):
+    """icellflag stored as floating point is converted to int; a NaN or a
+    value outside int's range cannot be converted and the read fails."""
+    p, _, _, _ = problem(oracle, qesmc, seed=4)
+    path = str(tmp_path / "badflag.nc")
+    scipy_workspace(path, p, 2, flag_type=flag_type, flag_fill=fill)
+    with pytest.raises(qesmc.QesmcError) as ei:
+        qesmc.workspace_read(path, velocity=False)
+    assert ei.value.code == qesmc.ERR_IO
+    # the same file with an integral value in range reads
+    scipy_workspace(path, p, 2, flag_type=flag_type, flag_fill=2.0)
+    r = qesmc.workspace_read(path, velocity=False)
+    assert r["icellflag"][r["icellflag"].size // 2] == 2
