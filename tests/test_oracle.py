@@ -453,3 +453,170 @@ def test_velocity_gradient_exact_for_linear_fields(oracle, stretched, dy):
         g = G[n].reshape(NZ, NY, NX)
         assert np.allclose(g[sel], expect[n], rtol=0, atol=2e-4), (n, np.abs(g[sel] - expect[n]).max())
         assert np.all(g[~sel] == 0)
+
+
+# ---------------------------------------------------------------------------
+# the post-correction divergence norm (oracle_div_linf)
+# ---------------------------------------------------------------------------
+def _with_dy(O, p, dy):
+    """The same problem with a different dy (build_city sets dy = dx)."""
+    return O.Problem(p.nx, p.ny, p.nz, p.dx, dy, p.dz_array, *p.coeffs, p.icellflag, p.u0, p.v0, p.w0)
+
+
+def _linear_faces(p, a, b, c, offsets=False, seed=0):
+    """u = a x_face, v = b y_face, w = c z_face (face (i,j,k) sits at x = i dx,
+    y = j dy, z = z_face[k]); with offsets, plus an arbitrary function of the
+    two other indices in each component (which every difference cancels)."""
+    O_zf = np.concatenate([[-p.dz_array[0]], [0.0], np.cumsum(p.dz_array[1:].astype(np.float64))])
+    K, J, I = np.meshgrid(np.arange(p.nz), np.arange(p.ny), np.arange(p.nx), indexing="ij")
+    u = a * I * p.dx
+    v = b * J * p.dy
+    w = c * O_zf[K]
+    if offsets:
+        rng = np.random.default_rng(seed)
+        u = u + rng.uniform(-3, 3, (p.nz, p.ny, 1))
+        v = v + rng.uniform(-3, 3, (p.nz, 1, p.nx))
+        w = w + rng.uniform(-3, 3, (1, p.ny, p.nx))
+    return [np.ascontiguousarray(x.ravel(), np.float32) for x in (u, v, w)], O_zf
+
+
+def _probe_cells(p, rng, nrandom=150):
+    """Interior fluid cells to probe: every one next to a zeroed coefficient
+    plus a random sample of the others."""
+    NX, NY, NZ = p.nx - 1, p.ny - 1, p.nz - 1
+    fl = p.icellflag.reshape(NZ, NY, NX)
+    co = [a.reshape(NZ, NY, NX) for a in p.coeffs]
+    sel = np.zeros_like(fl, bool)
+    sel[1:NZ - 2, 1:NY - 2, 1:NX - 2] = True
+    sel &= fl == 1
+    walled = sel & np.any([x == 0 for x in co], axis=0)
+    plain = np.argwhere(sel & ~walled)
+    picks = [tuple(x) for x in np.argwhere(walled)]
+    picks += [tuple(x) for x in plain[rng.choice(len(plain), min(nrandom, len(plain)), replace=False)]]
+    return picks
+
+
+def _single_cell_div(O, p, u, v, w, cell, physical):
+    """oracle_div_linf with only `cell` (k, j, i) marked fluid: |div| of that cell alone."""
+    k, j, i = cell
+    NX, NY = p.nx - 1, p.ny - 1
+    flag = np.zeros(p.ncell, np.int32)
+    flag[i + j * NX + k * NX * NY] = 1
+    q = O.Problem(p.nx, p.ny, p.nz, p.dx, p.dy, p.dz_array, *p.coeffs, flag, p.u0, p.v0, p.w0)
+    return O.div_linf(q, u, v, w, physical=physical)
+
+
+def test_div_linf_physical_operator_closed_form(oracle):
+    """Physical divergence of u = a x, v = b y, w = c z on a stretched grid
+    with dx != dy and buildings: exactly a + b + c in every interior fluid
+    cell (wall-adjacent ones included, the physical operator ignores the
+    coefficients), with arbitrary offsets that the differences cancel.  A
+    dropped /dz_k, a swapped dx/dy, a sign error or a shifted face index in
+    any term changes the value.  Single-cell flag arrays isolate each cell."""
+    O = oracle
+    p = _with_dy(O, small_city(O, 26, 20, 12, dx=2.0), 3.0)
+    a, b, c = 0.75, -0.5, 1.25
+    (u, v, w), _ = _linear_faces(p, a, b, c, offsets=True, seed=5)
+    rng = np.random.default_rng(1)
+    cells = _probe_cells(p, rng)
+    assert len(cells) > 200
+    for cell in cells:
+        d = _single_cell_div(O, p, u, v, w, cell, True)
+        assert abs(d - abs(a + b + c)) <= 2e-5, (cell, d)
+    # over the whole domain the norm is the same value
+    assert abs(O.div_linf(p, u, v, w, physical=True) - 1.5) <= 2e-5
+
+
+def test_div_linf_reference_operator_closed_form(oracle):
+    """Reference operator R(u)/(-2) (Solver_CPU_RB.cpp:56-80) of the same
+    linear field: with e..n either 0 or their normalised values 1/dx^2 (g, h
+    too: quirk A2) and 1/(dz_k (dz_k + dz_k+-1)/2) (Wall.cpp:827-846), each
+    axis contributes
+        x: a ((i+1) [e != 0] - i [f != 0])
+        y: b (dy/dx)^2 ((j+1) [g != 0] - j [h != 0])
+        z: c (z_face[k+1] [m != 0] - z_face[k] [n != 0])
+    so an interior cell gives a + b (dy/dx)^2 + c dz_k (the vertical term is
+    the plain difference of w, quirk A1).  A swapped e/f (or g/h, m/n), a
+    dropped dx or dz factor or a wrong face index changes the value at the
+    wall-adjacent cells, which are all probed."""
+    O = oracle
+    p = _with_dy(O, small_city(O, 26, 20, 12, dx=2.0), 3.0)
+    a, b, c = 0.75, -0.5, 1.25
+    (u, v, w), zf = _linear_faces(p, a, b, c)
+    NX, NY, NZ = p.nx - 1, p.ny - 1, p.nz - 1
+    e, f, g, h, m, n = (x.reshape(NZ, NY, NX) != 0 for x in p.coeffs)
+    r2 = (p.dy / p.dx) ** 2
+    rng = np.random.default_rng(2)
+    cells = _probe_cells(p, rng)
+    checked_walls = 0
+    for (k, j, i) in cells:
+        expect = (a * ((i + 1) * e[k, j, i] - i * f[k, j, i]) + b * r2 * ((j + 1) * g[k, j, i] - j * h[k, j, i])
+                  + c * (zf[k + 1] * m[k, j, i] - zf[k] * n[k, j, i]))
+        d = _single_cell_div(O, p, u, v, w, (k, j, i), False)
+        assert abs(d - abs(expect)) <= 1e-5 * max(1.0, abs(expect)), ((k, j, i), d, expect)
+        checked_walls += not (e[k, j, i] and f[k, j, i] and g[k, j, i] and h[k, j, i] and m[k, j, i] and n[k, j, i])
+    assert checked_walls > 100
+    # interior cells: a + b (dy/dx)^2 + c dz_k
+    dzk = p.dz_array[5]
+    d = _single_cell_div(O, p, u, v, w, (5, 3, 3), False)
+    assert abs(d - abs(a + b * r2 + c * dzk)) <= 1e-5
+
+
+def test_div_linf_matches_the_fp64_divergence_source(oracle):
+    """On an arbitrary field the reference-operator norm equals max |R| / 2 of
+    the FP64 divergence source (oracle_divergence_f64, pinned by the
+    reference's stored outputs and the divergence-free invariant) over the
+    interior fluid cells, bitwise, and the numpy restatement ref_operator_div
+    to rounding."""
+    O = oracle
+    p = small_city(O, 30, 24, 14)
+    rng = np.random.default_rng(9)
+    u, v, w = (rng.uniform(-5, 5, p.nface).astype(np.float32) for _ in range(3))
+    q = O.Problem(p.nx, p.ny, p.nz, p.dx, p.dy, p.dz_array, *p.coeffs, p.icellflag, u, v, w)
+    R = O.divergence_f64(q)
+    NX, NY, NZ = p.nx - 1, p.ny - 1, p.nz - 1
+    fl = p.icellflag.reshape(NZ, NY, NX)
+    sel = np.zeros_like(fl, bool)
+    sel[1:NZ - 2, 1:NY - 2, 1:NX - 2] = True
+    sel &= fl == 1
+    expect = np.abs(R.reshape(NZ, NY, NX)[sel]).max() / 2.0
+    got = O.div_linf(p, u, v, w, physical=False)
+    assert got == expect
+    D = ref_operator_div(p, u, v, w)
+    assert abs(np.abs(D[sel]).max() - got) <= 1e-12 * got
+
+
+# ---------------------------------------------------------------------------
+# rotated buildings (RectangularBuilding vertex list + PNPOLY)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("rot", [15.0, 30.0, 45.0, 120.0, 200.0, 333.0])
+def test_rotated_building_matches_half_plane_test(oracle, rot):
+    """A rectangle with corner (x0, y0), side L along (cos r, sin r) and W
+    along (-sin r, cos r) (RectangularBuilding.h:86-103): a cell centre is
+    inside iff both of its coordinates along those axes lie in [0, L] and
+    [0, W].  Cells whose centre is within 1e-3 m of an edge are skipped (the
+    crossing-number test decides those by float rounding); the layers are
+    k in [1, k_end) with z[k_end - 1] <= H < z[k_end] (PolyBuilding.cpp:
+    160-175)."""
+    O = oracle
+    ncx, ncy, dx = 60, 50, 1.5
+    dz = O.dz_stretched(1.0 * 1.04 ** np.arange(16))
+    x0, y0, L, W, H = 40.0, 25.0, 23.0, 11.0, 9.3
+    p = O.build_city(ncx, ncy, dx, dz, [(x0, y0, L, W, H, 0.0, rot)], (0, 3.0, 10.0, 0.1, 270.0))
+    NX, NY, NZ = ncx, ncy, dz.size
+    fl = p.icellflag.reshape(NZ, NY, NX)
+    _, z = O.vertical_grid(dz)
+    r = np.deg2rad(rot)
+    J, I = np.meshgrid(np.arange(NY), np.arange(NX), indexing="ij")
+    px, py = (I + 0.5) * dx - x0, (J + 0.5) * dx - y0
+    s = px * np.cos(r) + py * np.sin(r)
+    t = -px * np.sin(r) + py * np.cos(r)
+    inside = (s >= 0) & (s <= L) & (t >= 0) & (t <= W)
+    near = (np.minimum.reduce([np.abs(s), np.abs(s - L), np.abs(t), np.abs(t - W)]) < 1e-3)
+    k_end = int(np.argmax(z[1:] > H)) + 1
+    assert 2 < k_end < NZ - 1
+    for k in range(1, NZ - 1):
+        want = inside & (k < k_end)
+        got = fl[k] == 0
+        assert np.array_equal(got[~near], want[~near]), (k, np.argwhere(got != want)[:5])
+    assert inside.sum() > 100  # about L W / dx^2 = 112 cells per layer at dx 1.5: not vacuous
