@@ -463,7 +463,7 @@ def _with_dy(O, p, dy):
     return O.Problem(p.nx, p.ny, p.nz, p.dx, dy, p.dz_array, *p.coeffs, p.icellflag, p.u0, p.v0, p.w0)
 
 
-def _linear_faces(p, a, b, c, offsets=False, seed=0):
+def _linear_div_faces(p, a, b, c, offsets=False, seed=0):
     """u = a x_face, v = b y_face, w = c z_face (face (i,j,k) sits at x = i dx,
     y = j dy, z = z_face[k]); with offsets, plus an arbitrary function of the
     two other indices in each component (which every difference cancels)."""
@@ -516,7 +516,7 @@ def test_div_linf_physical_operator_closed_form(oracle):
     O = oracle
     p = _with_dy(O, small_city(O, 26, 20, 12, dx=2.0), 3.0)
     a, b, c = 0.75, -0.5, 1.25
-    (u, v, w), _ = _linear_faces(p, a, b, c, offsets=True, seed=5)
+    (u, v, w), _ = _linear_div_faces(p, a, b, c, offsets=True, seed=5)
     rng = np.random.default_rng(1)
     cells = _probe_cells(p, rng)
     assert len(cells) > 200
@@ -542,7 +542,7 @@ def test_div_linf_reference_operator_closed_form(oracle):
     O = oracle
     p = _with_dy(O, small_city(O, 26, 20, 12, dx=2.0), 3.0)
     a, b, c = 0.75, -0.5, 1.25
-    (u, v, w), zf = _linear_faces(p, a, b, c)
+    (u, v, w), zf = _linear_div_faces(p, a, b, c)
     NX, NY, NZ = p.nx - 1, p.ny - 1, p.nz - 1
     e, f, g, h, m, n = (x.reshape(NZ, NY, NX) != 0 for x in p.coeffs)
     r2 = (p.dy / p.dx) ** 2
