@@ -1,25 +1,32 @@
 """Benchmark of the MI355X mass-consistency solve (one GPU).
 
 A step is one complete Solver::solve of the reference schedule on the
-400 x 400 x 120 synthetic city: stencil encoding, lambda
-reset, divergence R, 500 red-black SOR iterations, velocity correction and
-solid zeroing, with every input already resident in HBM.  `value` is GLUPS
-(cell updates per second / 1e9) over the timed steps.
+1000 x 1000 x 150 synthetic city (151 M cells, 3000 lognormal buildings,
+wind from 270 deg): stencil encoding, lambda reset, divergence R, 500
+red-black SOR iterations, velocity correction and solid zeroing, with every
+input already resident in HBM.  `value` is GLUPS (cell updates per second /
+1e9) over the timed steps.
 
   python bench.py [--steps K] [--warmup W] [--gpus 1]
 
 Further legs reported in the same JSON line:
-  time_to_tol  128 x 128 x 64 held-out cities (the warm-start workload):
-               iterations and device ms until the velocity field is within
-               1e-4 m/s (L_inf) of the FP64 converged oracle, cold start
-               versus the trained U-Net's lambda0; U-Net MFMA throughput.
-  batch        64 cities of uneven size (64-256 x 64-256 x 64-96 cells),
-               wind directions 5.625 deg apart, 500 SOR iterations each:
-               one batched launch per colour pass (HIP graph replay) versus
-               the instances solved one after another.
-  cpu_baseline the oracle's fp32 red-black restatement on the host cores
-               (bounded sample); the same leg runs the FP64 oracle that the
-               time_to_tol leg checks the GPU fields against.
+  cpu_baseline  the oracle's fp32 red-black restatement on the host cores,
+                a bounded sample of the same 1000 x 1000 x 150 solve
+                (OMP_PROC_BIND=close, OMP_PLACES=cores).
+  city1000_tol  time to tolerance of the 8-direction ensemble on the same
+                city against the FP64 fixture tests/golden/city1000_fp64.npz:
+                iterations and device ms until the sampled velocity is within
+                1e-4 m/s of the FP64 solution, cold and from the
+                superposition of the two basis directions.
+  city400       the reference schedule on the 400 x 400 x 120 city.
+  time_to_tol   128 x 128 x 64 held-out cities (the warm-start workload):
+                iterations and device ms until the velocity field is within
+                1e-4 m/s (L_inf) of the FP64 converged oracle, cold start
+                versus the trained U-Net's lambda0; U-Net MFMA throughput.
+  batch         64 cities of uneven size (64-256 x 64-256 x 64-96 cells),
+                wind directions 5.625 deg apart, 500 SOR iterations each:
+                one batched launch per colour pass (HIP graph replay) versus
+                the instances solved one after another.
 
 Prints one JSON line.  --gpus above 1 is refused: this build runs on one GPU.
 """
@@ -32,6 +39,10 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "qes-public_amd"))
+# CPU baseline: one OpenMP thread per physical core, packed; the OpenMP
+# runtime reads these when the oracle library is first loaded
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 import numpy as np  # noqa: E402
 
@@ -103,38 +114,69 @@ def _cpu_model():
     return None
 
 
-def cpu_baseline(city, budget_s):
+def physical_cores():
+    """Physical cores among the CPUs this process may run on (sibling
+    hyperthreads counted once), from sysfs topology."""
+    seen = set()
+    try:
+        cpus = sorted(os.sched_getaffinity(0))
+    except AttributeError:
+        cpus = list(range(os.cpu_count() or 1))
+    for c in cpus:
+        base = "/sys/devices/system/cpu/cpu%d/topology/" % c
+        try:
+            with open(base + "core_id") as fh:
+                core = fh.read().strip()
+            with open(base + "physical_package_id") as fh:
+                pkg = fh.read().strip()
+        except OSError:
+            core, pkg = str(c), "0"
+        seen.add((pkg, core))
+    return len(seen), len(cpus)
+
+
+def cpu_baseline_resident(ctx, desc, samples=5, per_sample=4, serial_iters=1):
     """The oracle's fp32 red-black restatement of Solver_CPU_RB on the host
-    cores (OpenMP, all threads the runtime gives it): R once, then five timed
-    samples of 100 SOR iterations (fewer when the budget would be exceeded);
-    `value` is the median sample's rate.  Also the serial lexicographic
-    Solver_CPU restatement on one thread over a short sample."""
+    cores (OpenMP; OMP_PROC_BIND=close, OMP_PLACES=cores) on the resident
+    city of ctx, copied back from the device: R once, then `samples` timed
+    samples of `per_sample` SOR iterations; `value` is the median sample's
+    rate (per-iteration cost is the same at every iteration, so the rate
+    extrapolates to the 500-iteration schedule).  Also the serial
+    lexicographic Solver_CPU restatement on one thread."""
+    city = ctx.get_inputs()
     O = _oracle()
     p = O.Problem(**city.arrays())
+    del city
+    t0 = time.perf_counter()
     R = O.divergence_f32(p)
-    per_sample = max(10, min(100, int(100 * budget_s / 12.0)))
+    r_s = time.perf_counter() - t0
     lam, rates = None, []
-    t_all = time.perf_counter()
-    for _ in range(5):
+    for _ in range(samples):
         t0 = time.perf_counter()
         lam, it, _ = O.sor_rb_f32(p, R, lam, itermax=per_sample, tol=-1.0)
         rates.append(p.interior_lups() * it / (time.perf_counter() - t0) / 1e9)
-        if time.perf_counter() - t_all > budget_s:
-            break
     threads = O.num_threads()
     O.set_num_threads(1)
     try:
         t0 = time.perf_counter()
-        _, it_s, _ = O.sor_serial_f32(p, R, None, itermax=5, tol=-1.0)
+        _, it_s, _ = O.sor_serial_f32(p, R, None, itermax=serial_iters, tol=-1.0)
         serial = p.interior_lups() * it_s / (time.perf_counter() - t0) / 1e9
     finally:
         O.set_num_threads(threads)
-    return {"value": round(float(np.median(rates)), 4), "unit": "GLUPS", "cores": threads, "kind": "port",
+    phys, logical = physical_cores()
+    glups = float(np.median(rates))
+    its = 500
+    t_ref_ms = (r_s + p.interior_lups() * its / (glups * 1e9)) * 1e3
+    return {"value": round(glups, 4), "unit": "GLUPS", "cores": threads, "kind": "port",
+            "physical_cores_available": phys, "logical_cpus_available": logical,
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND"), "omp_places": os.environ.get("OMP_PLACES"),
             "samples_glups": [round(r, 4) for r in rates], "cpu_model": _cpu_model(),
             "serial_single_thread_glups": round(serial, 4),
-            "sample": f"400x400x120 domain, median of {len(rates)} samples of {per_sample} SOR iterations "
-                      f"(oracle/qesmc_oracle.c Solver_CPU_RB restatement, OpenMP, {threads} threads); serial "
-                      f"Solver_CPU restatement on 1 thread over 5 iterations"}
+            "t_ref_ms_extrapolated": round(t_ref_ms, 1),
+            "sample": "%s: R once (%.2f s), then the median of %d samples of %d SOR iterations "
+                      "(oracle/qesmc_oracle.c Solver_CPU_RB restatement, OpenMP, %d threads); t_ref_ms_extrapolated "
+                      "= R + 500 iterations at that rate (extrapolated, not run); serial Solver_CPU restatement "
+                      "on 1 thread over %d iteration(s)" % (desc, r_s, samples, per_sample, threads, serial_iters)}
 
 
 WEIGHTS = os.path.join(ROOT, "qes-public_amd", "qesmc", "weights", "unet_128x128x64.npz")
@@ -145,7 +187,8 @@ def _linf(a, b):
     return float(np.abs(a.astype(np.float64) - b).max())
 
 
-STOP_TOLS = (1e-3, 3e-4, 1e-4, 6e-5, 5e-5, 4e-5, 3e-5)
+# (name, qesmc_params.stop_extend) at tol 1e-4
+STOP_RULES = (("reference_tol_1e-4", 0.0), ("production_tol_1e-4_extend_0.3", 0.3))
 
 
 def time_to_tol_leg(seeds, fp64_ref, tol_ms=1e-4, chunk=25, cap=6000):
@@ -157,10 +200,10 @@ def time_to_tol_leg(seeds, fp64_ref, tol_ms=1e-4, chunk=25, cap=6000):
     500 iterations).  Pass 2 times that many iterations from a fresh start:
     R + SOR + correction on the device (qesmc_result.ms_total), plus the
     U-Net inference for the warm start.  Without FP64 fields the stop rule
-    max|dlambda| <= 1e-4 is used instead.  With them, the library's stop rule
-    (max|dlambda| <= tol) is also run at each of STOP_TOLS, recording where
-    it stops and its distance to the FP64 field: which tol a production run
-    needs for the 1e-4 m/s criterion."""
+    max|dlambda| <= 1e-4 is used instead.  With them, the device's stop
+    rules are also run at tol 1e-4 (STOP_RULES: the reference's, and the
+    production rule that runs 30 % more iterations after it), recording
+    where each stops, its device ms and its distance to the FP64 field."""
     from qesmc import warmstart as W
 
     if not os.path.exists(WEIGHTS):
@@ -202,18 +245,18 @@ def time_to_tol_leg(seeds, fp64_ref, tol_ms=1e-4, chunk=25, cap=6000):
                 if mode == "warm":
                     row[mode].update(unet_ms=round(unet_ms, 3), unet_tflops=round(tflops, 1))
                 if fp64_ref is not None:
-                    # the library's own stop rule (max|dlambda| <= tol, as the
-                    # reference's) at several tol: where it stops, and how far
-                    # from the FP64 field that is
+                    # the device's stop rules at tol 1e-4: the reference's
+                    # (max|dlambda| <= tol) and the production one (then 30 %
+                    # more iterations): where they stop, what it costs, and how
+                    # far from the FP64 field that is
                     row[mode]["stop_rule"] = {}
-                    for stol in STOP_TOLS:
+                    for name, ext in STOP_RULES:
                         ctx.set_lambda(None)
-                        if mode == "warm":
-                            net.predict(ctx)
-                        r = ctx.run(itermax=cap, tol=stol)
+                        pred_ms = net.predict(ctx)[0] if mode == "warm" else 0.0
+                        r = ctx.run(itermax=cap, tol=1e-4, stop_extend=ext)
                         u, v, w = ctx.get_velocity()
-                        row[mode]["stop_rule"][str(stol)] = {
-                            "iterations": r["iterations"],
+                        row[mode]["stop_rule"][name] = {
+                            "iterations": r["iterations"], "ms": round(r["ms_total"] + pred_ms, 3),
                             "err": max(_linf(u, u64), _linf(v, v64), _linf(w, w64))}
             rows.append(row)
     net.close()
@@ -228,11 +271,15 @@ def time_to_tol_leg(seeds, fp64_ref, tol_ms=1e-4, chunk=25, cap=6000):
         stop = {}
         for mode in ("cold", "warm"):
             stop[mode] = {}
-            for stol in STOP_TOLS:
-                v = [r[mode]["stop_rule"][str(stol)] for r in rows]
-                stop[mode][str(stol)] = {"mean_iterations": float(np.mean([x["iterations"] for x in v])),
-                                         "max_err": max(x["err"] for x in v),
-                                         "cities_within_tol_ms": sum(x["err"] <= tol_ms for x in v)}
+            for name, _ in STOP_RULES:
+                v = [r[mode]["stop_rule"][name] for r in rows]
+                stop[mode][name] = {"mean_iterations": float(np.mean([x["iterations"] for x in v])),
+                                    "mean_ms": round(float(np.mean([x["ms"] for x in v])), 3),
+                                    "max_err": max(x["err"] for x in v),
+                                    "cities_within_tol_ms": sum(x["err"] <= tol_ms for x in v)}
+        prod = STOP_RULES[1][0]
+        stop["speedup_warm_vs_cold_production_rule"] = round(stop["cold"][prod]["mean_ms"]
+                                                             / stop["warm"][prod]["mean_ms"], 3)
     tfl = mean("warm", "unet_tflops")
     return {
         "workload": "%d held-out seeded cities, 128x128x64 cells, dx=2 m, dz 1 m x1.02, 12-30 buildings, log law, "
@@ -329,128 +376,202 @@ def large_city(dirs=8):
     return ncx, ncy, dz, blds, u_ref, winds
 
 
-def _tol_run(ctx, ref, chunk, cap, tol_ms):
-    """Run SOR on ctx in chunks from its current lambda until its velocity is
-    within tol_ms (L_inf) of ref's; returns (iterations, device ms).  The ms
-    are the chunks' SOR time plus one R + correction (the chunk runs repeat
-    those only to read the field back)."""
-    its, ms_sor, extra = 0, 0.0, None
+FIXTURE_C5 = os.path.join(ROOT, "tests", "golden", "city1000_fp64.npz")
+
+
+def c5_fixture_probes(F, nx, ny):
+    """Face indices (i + j nx + k nx ny) of the FP64 fixture's samples of the
+    1000 x 1000 x 150 city (x-z slices of rows xz_rows, y-z slices of columns
+    yz_cols, layers k < ks, and random faces pts) and the FP64 u, v, w of the
+    two basis directions there: {270: (3, n), 180: (3, n)} float64."""
+    ks = int(F["ks"])
+    faces, vals = [], {270: [], 180: []}
+    K, I = np.meshgrid(np.arange(ks, dtype=np.int64), np.arange(nx, dtype=np.int64), indexing="ij")
+    for r, j in enumerate(F["xz_rows"]):
+        faces.append((I + int(j) * nx + K * nx * ny).ravel())
+        for tag in vals:
+            vals[tag].append(F["xz_%d" % tag][r].reshape(3, -1))
+    K, J = np.meshgrid(np.arange(ks, dtype=np.int64), np.arange(ny, dtype=np.int64), indexing="ij")
+    for r, i in enumerate(F["yz_cols"]):
+        faces.append((int(i) + J * nx + K * nx * ny).ravel())
+        for tag in vals:
+            vals[tag].append(F["yz_%d" % tag][r].reshape(3, -1))
+    faces.append(F["pts"].astype(np.int64))
+    for tag in vals:
+        vals[tag].append(F["pts_%d" % tag])
+    return np.concatenate(faces), {t: np.concatenate(v, axis=1).astype(np.float64) for t, v in vals.items()}
+
+
+def c5_reference(ref, wind_deg):
+    """FP64 sampled field of any direction: the problem is linear in the
+    inflow and a single-sensor inflow from `wind_deg` is cos(t) x the 270 deg
+    inflow + sin(t) x the 180 deg one, t = 270 - wind_deg."""
+    th = math.radians(270.0 - wind_deg)
+    return math.cos(th) * ref[270] + math.sin(th) * ref[180]
+
+
+def probe_error(ctx, ref_dir):
+    """Sampled L_inf (u, v, w) distance of ctx's corrected field to ref_dir."""
+    got = np.stack(Q.get_probes(ctx)).astype(np.float64)
+    d = np.abs(got - ref_dir)
+    return float(np.inf) if not np.all(np.isfinite(d)) else float(d.max())
+
+
+def tol_search(work, save, ref_dir, tol_ms, coarse, fine, cap):
+    """Iterations from work's current lambda until its sampled velocity is
+    within tol_ms of ref_dir: chunks of `coarse` iterations (lambda saved in
+    `save` before each), then, from the last chunk's start, chunks of `fine`.
+    Returns (iterations or None, SOR ms per iteration, R + correction ms,
+    sampled error at the stop, errors at the coarse checkpoints)."""
+    its, run_its, ms_sor, extra, hist = 0, 0, 0.0, 0.0, []
+    r = work.run(itermax=0, tol=-1.0)
+    extra = r["ms_total"]
+    e = probe_error(work, ref_dir)
+    hist.append((0, e))
+    if e <= tol_ms:
+        return 0, 0.0, extra, e, hist
     while its < cap:
-        r = ctx.run(itermax=chunk, tol=-1.0)
-        its += chunk
+        Q.lambda_combine(save, [work], np.array([1.0], np.float32))
+        r = work.run(itermax=coarse, tol=-1.0)
+        its += coarse
+        run_its += coarse
         ms_sor += r["ms_sor"]
-        if extra is None:
-            extra = r["ms_total"] - r["ms_sor"]
-        if Q.velocity_linf_diff(ctx, ref) <= tol_ms:
-            return its, ms_sor + extra
-    return None, ms_sor + (extra or 0.0)
+        extra = r["ms_total"] - r["ms_sor"]
+        e = probe_error(work, ref_dir)
+        hist.append((its, e))
+        print(f"  tol search: {its} its, sampled error {e:.3e} m/s", file=sys.stderr, flush=True)
+        if e <= tol_ms:
+            Q.lambda_combine(work, [save], np.array([1.0], np.float32))
+            its -= coarse
+            while True:
+                r = work.run(itermax=fine, tol=-1.0)
+                its += fine
+                run_its += fine
+                ms_sor += r["ms_sor"]
+                e = probe_error(work, ref_dir)
+                if e <= tol_ms:
+                    return its, ms_sor / run_its, extra, e, hist
+    return None, ms_sor / max(run_its, 1), extra, e, hist
 
 
-def large_leg(dirs, measured, ref_iters, cpu_iters, tol_ms=1e-4):
-    """Time-to-tolerance of a wind-direction ensemble on the 1000 x 1000 x 150
-    city.  An FP64 CPU solve of 151 M cells is out of reach (hours), so each
-    measured direction's reference is the GPU's own fp32 field after
-    ref_iters iterations: at this size the field stops changing (beyond fp32
-    noise of ~6e-5 m/s between checkpoints 10000 iterations apart) after
-    about 60000 iterations.  T_tol = device ms until the velocity is within
-    tol_ms (L_inf, u, v, w) of the reference:
-      cold        from lambda = 0 (checked every 2500 iterations);
-      superposed  directions 270 and 180 deg (wind along +x and +y) cold; any
-                  other direction from cos(theta) lambda_270 + sin(theta)
-                  lambda_180 (checked every 25 iterations).
-    Only the `measured` directions are solved (each costs two long runs); the
-    `dirs`-direction ensemble total is extrapolated from them and reported as
-    such.  Directions run one after another: at 151 M cells one problem fills
-    the GPU.  Also the classic-layout path's time per iteration (the
-    reference GPU algorithm) and a CPU sample of the oracle."""
-    ncx, ncy, dz, blds, u_ref, _ = large_city(dirs)
-    order = [270.0, 180.0] + [w for w in measured if w not in (270.0, 180.0)]
-    out = {"workload": "1000x1000x150 cells (151 M), dx=dy=1 m, dz 1 m x1.02, %d lognormal buildings "
-                       "(median 20 m, sigma 0.6, cap 120 m, footprints U(6,16) m), log law u*=0.5 m/s z0=0.5 m, "
-                       "ensemble of %d directions %.0f deg apart" % (len(blds), dirs, 360.0 / dirs),
-           "criterion": "L_inf(u,v,w) <= %g m/s of the direction's fp32 field after %d iterations"
-                        % (tol_ms, ref_iters)}
-    ctx = [Q.Context(0) for _ in range(4)]  # reference, work, basis x, basis y
-    ref, work, bx, by = ctx
+def c5_tol_leg(tol_ms=1e-4, dirs=8, coarse=2000, fine=100, cap=120000, cpu_ms_per_it=None, cold_dirs=(270.0,),
+               warm=True):
+    """Time to tolerance of the `dirs`-direction ensemble on the 1000 x 1000 x
+    150 city, against the FP64 fixture (sampled: two x-z and two y-z slices of
+    the lowest 100 layers and 131072 random faces; the fixture's FP64 fields
+    are the fixed point of the oracle's FP64 red-black SOR).  T_tol = device
+    ms of R + SOR + correction (+ the U-Net call) until the sampled velocity
+    is within tol_ms of the FP64 solution:
+      cold        `cold_dirs` from lambda = 0;
+      warm        the two basis directions (270 and 180 deg) from the U-Net's
+                  lambda0 (the shipped weights, trained on 128 x 128 x 64
+                  cities, instantiated at this size) -- or cold when warm is
+                  False;
+      superposed  every other direction from cos(t) lambda_270 + sin(t)
+                  lambda_180 (the basis solutions at their T_tol), checked
+                  every 25 iterations.
+    The cold ensemble total uses the mean cold time of `cold_dirs` for every
+    direction (extrapolated, stated as such)."""
+    if not os.path.exists(FIXTURE_C5):
+        return {"skipped": "FP64 fixture not found: " + os.path.relpath(FIXTURE_C5, ROOT)}
+    F = np.load(FIXTURE_C5)
+    ncx, ncy, dz, blds, u_ref, winds = large_city(dirs)
+    if not (np.array_equal(F["dz_array"], dz) and np.array_equal(F["buildings"], np.array(blds, np.float32))):
+        return {"skipped": "fixture city differs from the generator's"}
+    faces, ref = c5_fixture_probes(F, ncx + 1, ncy + 1)
+    out = {"workload": "1000x1000x150 cells (151 M), dx=dy=1 m, dz 1 m x1.02, %d lognormal buildings, log law "
+                       "u*=0.5 m/s z0=0.5 m, ensemble of %d directions %.0f deg apart" % (len(blds), dirs, 360.0 / dirs),
+           "criterion": "sampled L_inf(u,v,w) <= %g m/s of the FP64 solution (%d sampled faces x 3 components: "
+                        "2 x-z and 2 y-z slices of layers k < %d, %d random faces); basis solves checked every %d "
+                        "iterations then every %d, superposed every 25" % (tol_ms, faces.size, int(F["ks"]),
+                                                                            F["pts"].size, coarse, fine),
+           "fp64_fixture": {"iterations": [int(F["iterations_270"]), int(F["iterations_180"])],
+                            "final_max_dlambda": [float(F["final_dlambda_270"]), float(F["final_dlambda_180"])],
+                            "check_max_dlambda_omega_1.78": [float(F["check_dlambda_270"]),
+                                                             float(F["check_dlambda_180"])]}}
+    net = None
+    if warm:
+        from qesmc import warmstart as W
+
+        if os.path.exists(WEIGHTS):
+            net = W.load(WEIGHTS, size=(ncx, ncy, dz.size))
+    ctx = [Q.Context(0) for _ in range(4)]  # work, save, basis 270, basis 180
+    work, save, bx, by = ctx
     try:
-        t0 = time.perf_counter()
         for c in ctx:
             c.set_city(ncx, ncy, 1.0, dz, blds, (1, u_ref, 10.0, 0.5, 270.0))
-        out["device_city_build_s_per_context"] = round((time.perf_counter() - t0) / 4, 3)
-        lups = ref.city.interior_lups()
-        # tiling chosen by the autotuner for this domain (kept per context)
-        t0 = time.perf_counter()
-        best = None
-        for c in ctx:
-            best = Q.autotune(c, iterations=20)
-        out["autotuned_tile"] = [best.tile_k, best.tile_v, best.wg_per_cu]
-        out["autotune_s"] = round(time.perf_counter() - t0, 2)
+        Q.set_probes(work, faces)
+        lups = work.city.interior_lups()
         r = work.run(itermax=100, tol=-1.0, path=Q.PATH_PLAIN)
         plain_ms_it = r["ms_sor"] / 100
-        work.set_lambda(None)
-        r = work.run(itermax=100, tol=-1.0)
-        split_ms_it = r["ms_sor"] / 100
-        out["ms_per_iteration"] = {"plain": round(plain_ms_it, 4), "split": round(split_ms_it, 4)}
-        out["glups"] = {"plain": round(lups / plain_ms_it / 1e6, 1), "split": round(lups / split_ms_it / 1e6, 1)}
-        print(f"large: plain {plain_ms_it:.3f} ms/it, split {split_ms_it:.3f} ms/it", file=sys.stderr, flush=True)
         rows = []
-        for wd in order:
-            inflow = (1, u_ref, 10.0, 0.5, wd)
-            ref.set_city_inflow(inflow)
-            ref.set_lambda(None)
-            ref.run(itermax=ref_iters, tol=-1.0)
-            work.set_city_inflow(inflow)
+
+        def basis_solve(wd, mode):
+            work.set_city_inflow((1, u_ref, 10.0, 0.5, wd))
             work.set_lambda(None)
-            c_its, c_ms = _tol_run(work, ref, 2500, ref_iters, tol_ms)
-            row = {"wind_deg": wd, "cold_iterations": c_its, "cold_ms": round(c_ms, 1)}
-            if wd in (270.0, 180.0):
-                Q.lambda_combine(bx if wd == 270.0 else by, [work], np.array([1.0], np.float32))
-                row["role"] = "basis"
-            else:
-                th = math.radians(270.0 - wd)
-                tc = time.perf_counter()
-                Q.lambda_combine(work, [bx, by], np.array([math.cos(th), math.sin(th)], np.float32))
-                comb_ms = (time.perf_counter() - tc) * 1e3
-                w_its, w_ms = _tol_run(work, ref, 25, 20000, tol_ms)
-                row.update(role="superposed", superposed_iterations=w_its, superposed_ms=round(w_ms + comb_ms, 1))
+            pred_ms, tflops = net.predict(work) if mode == "warm" else (0.0, 0.0)
+            its, ms_it, extra, err, hist = tol_search(work, save, c5_reference(ref, wd), tol_ms, coarse, fine, cap)
+            row = {"wind_deg": wd, "start": mode, "iterations": its, "sampled_err": err,
+                   "ms": round(its * ms_it + extra + pred_ms, 1) if its is not None else None,
+                   "sor_ms_per_iteration": round(ms_it, 4),
+                   "err_at_checkpoints": [[i, round(e, 7)] for i, e in hist]}
+            if mode == "warm":
+                row.update(unet_ms=round(pred_ms, 1), unet_tflops=round(tflops, 1))
+            print(f"c5 tol: {wd:5.1f} deg {mode} {its} its {row['ms']} ms err {err:.2e}", file=sys.stderr, flush=True)
+            return row
+
+        if net is not None:
+            for wd in cold_dirs:
+                row = basis_solve(wd, "cold")
+                row["role"] = "cold reference"
+                rows.append(row)
+        for wd, basis in ((270.0, bx), (180.0, by)):
+            row = basis_solve(wd, "warm" if net is not None else "cold")
+            row["role"] = "basis"
             rows.append(row)
-            print(f"large: {wd:5.1f} deg cold {c_its} its {c_ms:.0f} ms, superposed "
-                  f"{row.get('superposed_iterations')} its {row.get('superposed_ms')} ms", file=sys.stderr, flush=True)
-        basis = [r_ for r_ in rows if r_["role"] == "basis"]
-        others = [r_ for r_ in rows if r_["role"] == "superposed"]
-        cold_mean = float(np.mean([r_["cold_ms"] for r_ in rows]))
-        cold_its_mean = float(np.mean([r_["cold_iterations"] or ref_iters for r_ in rows]))
-        sup_mean = float(np.mean([r_["superposed_ms"] for r_ in others])) if others else 0.0
-        cold_ens = dirs * cold_mean
-        sup_ens = sum(r_["cold_ms"] for r_ in basis) + (dirs - 2) * sup_mean
+            Q.lambda_combine(basis, [work], np.array([1.0], np.float32))
+        for wd in winds:
+            if wd in (270.0, 180.0):
+                continue
+            th = math.radians(270.0 - wd)
+            work.set_city_inflow((1, u_ref, 10.0, 0.5, wd))
+            tc = time.perf_counter()
+            Q.lambda_combine(work, [bx, by], np.array([math.cos(th), math.sin(th)], np.float32))
+            comb_ms = (time.perf_counter() - tc) * 1e3
+            its, ms_it, extra, err, _ = tol_search(work, save, c5_reference(ref, wd), tol_ms, 25, 25, 20000)
+            row = {"wind_deg": wd, "start": "superposed", "role": "superposed", "iterations": its, "sampled_err": err,
+                   "ms": round(its * ms_it + extra + comb_ms, 1) if its is not None else None}
+            rows.append(row)
+            print(f"c5 tol: {wd:5.1f} deg superposed {its} its {row['ms']} ms err {err:.2e}", file=sys.stderr,
+                  flush=True)
         out["per_direction"] = rows
-        out["ensemble_extrapolated"] = {
-            "note": "%d measured directions; the %d-direction totals use the mean cold time of the measured ones and "
-                    "the mean superposed time of the measured non-basis ones" % (len(rows), dirs),
-            "cold_ms": round(cold_ens, 1), "superposed_ms": round(sup_ens, 1),
-            "plain_path_cold_ms": round(dirs * cold_its_mean * plain_ms_it, 1),
-            "speedup_superposed_vs_cold": round(cold_ens / sup_ens, 2),
-            "speedup_superposed_vs_plain_cold": round(dirs * cold_its_mean * plain_ms_it / sup_ens, 2)}
-        if cpu_iters > 0:
-            city = ref.get_inputs()
-            O = _oracle()
-            p = O.Problem(**city.arrays())
-            del city
-            t0 = time.perf_counter()
-            R = O.divergence_f32(p)
-            O.sor_rb_f32(p, R, None, itermax=cpu_iters, tol=-1.0)
-            el = time.perf_counter() - t0
-            cpu_ms_it = el * 1e3 / cpu_iters
-            cpu_ens = dirs * cold_its_mean * cpu_ms_it
-            out["cpu_baseline"] = {"ms_per_iteration": round(cpu_ms_it, 2), "cores": O.num_threads(), "kind": "port",
-                                   "glups": round(lups / cpu_ms_it / 1e6, 3),
-                                   "sample": "R + %d SOR iterations (%.1f s), extrapolated to the ensemble's cold "
-                                             "iteration counts" % (cpu_iters, el),
-                                   "ensemble_cold_ms_extrapolated": round(cpu_ens, 0)}
-            out["ensemble_extrapolated"]["speedup_superposed_vs_cpu"] = round(cpu_ens / sup_ens, 1)
+        cold = [r_ for r_ in rows if r_["start"] == "cold"]
+        basis = [r_ for r_ in rows if r_.get("role") == "basis"]
+        sup = [r_ for r_ in rows if r_.get("role") == "superposed"]
+        if all(r_["ms"] is not None for r_ in cold + basis + sup) and cold:
+            cold_mean = float(np.mean([r_["ms"] for r_ in cold]))
+            cold_its = float(np.mean([r_["iterations"] for r_ in cold]))
+            ens_total = sum(r_["ms"] for r_ in basis) + sum(r_["ms"] for r_ in sup)
+            ens = {"ms": round(ens_total, 1), "measured": True,
+                   "method": "basis directions from %s, the others by superposition" % basis[0]["start"],
+                   "cold_ms_extrapolated": round(dirs * cold_mean, 1),
+                   "plain_path_cold_ms_extrapolated": round(dirs * (cold_its * plain_ms_it), 1),
+                   "speedup_vs_cold": round(dirs * cold_mean / ens_total, 2),
+                   "speedup_vs_plain_path_cold": round(dirs * cold_its * plain_ms_it / ens_total, 2),
+                   "note": "ensemble total measured (2 basis solves + %d superposed directions); the cold totals use "
+                           "the mean cold iterations of %s deg for every direction"
+                           % (len(sup), [r_["wind_deg"] for r_ in cold])}
+            if cpu_ms_per_it:
+                ens["cpu_cold_ms_extrapolated"] = round(dirs * cold_its * cpu_ms_per_it, 0)
+                ens["speedup_vs_cpu"] = round(dirs * cold_its * cpu_ms_per_it / ens_total, 1)
+            out["ensemble"] = ens
+        out["plain_ms_per_iteration"] = round(plain_ms_it, 4)
+        out["plain_glups"] = round(lups / plain_ms_it / 1e6, 1)
     finally:
         for c in ctx:
             c.close()
+        if net is not None:
+            net.close()
     return out
 
 
@@ -501,13 +622,84 @@ def tile_sweep_leg():
     return out
 
 
-def load_traffic(workload_key):
+def reference_schedule(ctx, params, steps, warmup):
+    """`warmup` untimed, then `steps` timed solves of the reference schedule
+    on the resident city: lambda reset, R, SOR, correction, zeroing (and the
+    stencil encoding, which params asks for).  Returns (wall s, results)."""
+    for _ in range(warmup):
+        ctx.set_lambda(None)
+        ctx.run(params)
+    results = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.set_lambda(None)
+        results.append(ctx.run(params))
+    return time.perf_counter() - t0, results
+
+
+def roofline_of(results, lups_per_iteration, workload_key):
+    """Roofline object of the dominant SOR kernel from the run's own HIP-event
+    timings: achieved = algorithmic bytes per launch / mean launch time."""
+    last = results[-1]
+    its = last["iterations"]
+    fused = last["path"] == Q.PATH_FUSED
+    launches = its if fused else 2 * its
+    ms_sor = float(np.mean([r["ms_sor"] for r in results]))
+    launch_ms = ms_sor / launches
+    bpl = bytes_per_lup(last)
+    per_launch_updates = lups_per_iteration / (1 if fused else 2)
+    achieved = bpl * per_launch_updates / (launch_ms * 1e-3) / 1e9
+    if last["path"] == Q.PATH_PLAIN:
+        kernel = "k_rb_plain"
+    elif fused:
+        kernel = "k_sor_fused"
+    else:
+        kernel = "k_sor_split<%s, %d>" % ("true" if last["compressed"] else "false", last["tile_v"])
+    traffic, source = load_traffic(kernel, workload_key)
+    ref36 = 36.0 * per_launch_updates / (launch_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "traffic_source": source, "bytes_per_lup": bpl, "avg_launch_us": round(launch_ms * 1e3, 2),
+            "updates_per_launch": int(per_launch_updates),
+            "at_36_bytes_per_lup": {"achieved": round(ref36, 1), "frac": round(ref36 / HBM_PEAK_GBS, 4),
+                                    "note": "six fp32 coefficients + R + lambda r/w per update, the reference "
+                                            "layout's algorithmic bytes; this kernel reads a 1-byte wall mask "
+                                            "instead of the coefficients, bitwise equivalent"}}, ms_sor
+
+
+def load_traffic(kernel, workload_key):
+    """HBM bytes per dispatch of `kernel` on `workload_key` from the committed
+    counter summary (profiles/traffic.json: rocprofv3 FETCH_SIZE x the copy
+    calibration factor + WRITE_SIZE)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as fh:
-            return json.load(fh).get(workload_key)
+            t = json.load(fh)
     except (OSError, ValueError):
-        return None
+        return None, None
+    ent = t.get(workload_key, {}).get(kernel)
+    if ent is None:
+        return None, None
+    return ent["bytes"], "profiles/" + ent["summary"] + " (not measured in this run)"
+
+
+def city400_leg(path, tiles, steps=5, warmup=1):
+    """The reference schedule on the 400 x 400 x 120 city."""
+    city, desc = city_400x400x120()
+    params = Q.default_params(itermax=500, tol=1e-9, path=path, flags=Q.RUN_REENCODE, **tiles)
+    with Q.Context(0) as ctx:
+        ctx.set_geometry(city)
+        ctx.set_inflow(city.u0, city.v0, city.w0)
+        wall, results = reference_schedule(ctx, params, steps, warmup)
+        div_ref = ctx.divergence_linf(False)
+        grad_ms = min(ctx.velocity_gradient(uniform_z=False, copy=False)[1] for _ in range(3))
+    its = results[-1]["iterations"]
+    roof, ms_sor = roofline_of(results, city.interior_lups(), "city400x400x120")
+    return {"workload": desc, "steps": steps, "t_ref_ms": round(wall / steps * 1e3, 3), "sor_ms": round(ms_sor, 3),
+            "glups": round(city.interior_lups() * its * steps / wall / 1e9, 3), "div_linf_ref_operator": div_ref,
+            "velocity_gradient": {"ms": round(grad_ms, 4), "bytes_per_cell": 52,
+                                  "achieved_gbs": round(52.0 * city.ncell / (grad_ms * 1e-3) / 1e9, 1)},
+            "roofline": roof}
 
 
 def main():
@@ -520,15 +712,16 @@ def main():
     ap.add_argument("--tile-k", type=int, default=0, help="k planes per SOR work-group (0: library choice)")
     ap.add_argument("--tile-v", type=int, default=0, help="split path: half-cells per lane (0: library choice)")
     ap.add_argument("--wg-per-cu", type=int, default=0, help="split path: work-groups per CU (0: library choice)")
-    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline sampling (0: skip)")
-    ap.add_argument("--ttol-cities", type=int, default=8, help="time_to_tol leg: held-out cities (0: skip)")
-    ap.add_argument("--batch", type=int, default=64, help="batch leg: instances (0: skip)")
-    ap.add_argument("--workload", choices=["city400", "city1000"], default="city400",
-                    help="city1000: only the 1000x1000x150 8-direction ensemble (minutes of GPU time)")
-    ap.add_argument("--large-dirs", type=int, default=8, help="ensemble size of the large workload")
-    ap.add_argument("--large-measure", default="270,180,45,135", help="directions solved (deg); 270 and 180 are the basis")
-    ap.add_argument("--large-ref-iters", type=int, default=100000, help="iterations of each direction's reference")
-    ap.add_argument("--large-cpu-iters", type=int, default=20)
+    ap.add_argument("--slab-planes", type=int, default=0,
+                    help="split path: k-slab of the interleaved red/black schedule (0: library choice, -1: whole "
+                         "passes)")
+    ap.add_argument("--legs", default="cpu,city1000_tol,city400,ttol,batch",
+                    help="extra legs after the headline (comma list of cpu, city1000_tol, city400, ttol, batch; "
+                         "'none' for the headline alone)")
+    ap.add_argument("--skip-headline", action="store_true",
+                    help="run only the legs named by --legs (no headline, no bench line; for profiling a leg)")
+    ap.add_argument("--ttol-cities", type=int, default=8, help="time_to_tol leg: held-out cities")
+    ap.add_argument("--batch", type=int, default=64, help="batch leg: instances")
     ap.add_argument("--tile-sweep", action="store_true",
                     help="only time a grid of SOR tilings on four domain sizes (one JSON line, about a minute)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -558,50 +751,32 @@ def main():
         print(json.dumps({"copy_calibration_bytes": n * 4}))
         return
 
-    if a.workload == "city1000":
-        measured = [float(v) for v in a.large_measure.split(",")]
-        out = large_leg(a.large_dirs, measured, a.large_ref_iters, a.large_cpu_iters)
-        print(json.dumps({"metric": METRIC, "workload": "city1000", "n_gpus": 1, "city1000": out}))
-        return
-
-    city, desc = city_400x400x120()
+    legs = [] if a.legs == "none" else [x for x in a.legs.split(",") if x]
     path = {"auto": Q.PATH_AUTO, "plain": Q.PATH_PLAIN, "split": Q.PATH_SPLIT, "fused": Q.PATH_FUSED}[a.path]
-    params = Q.default_params(itermax=a.itermax, tol=1e-9, path=path, flags=Q.RUN_REENCODE, tile_k=a.tile_k,
-                              tile_v=a.tile_v, wg_per_cu=a.wg_per_cu)
+    tiles = dict(tile_k=a.tile_k, tile_v=a.tile_v, wg_per_cu=a.wg_per_cu, slab_planes=a.slab_planes)
+    params = Q.default_params(itermax=a.itermax, tol=1e-9, path=path, flags=Q.RUN_REENCODE, **tiles)
+    ncx, ncy, dz, blds, u_ref, _ = large_city()
+    desc = ("1000x1000x150 cells (151 M), dx=dy=1 m, dz 1 m x1.02 geometric, %d lognormal buildings (median 20 m, "
+            "sigma 0.6, cap 120 m, footprints U(6,16) m), log law u*=0.5 m/s z0=0.5 m from 270 deg; R + %d red-black "
+            "SOR iterations + correction" % (len(blds), a.itermax))
+    if a.skip_headline:
+        out = {"metric": METRIC, "legs_only": legs}
+        run_legs(a, legs, out, path, tiles, None)
+        print(json.dumps(out))
+        return
+    cpu = None
     with Q.Context(0) as ctx:
-        ctx.set_geometry(city)
-        ctx.set_inflow(city.u0, city.v0, city.w0)
-        for _ in range(a.warmup):
-            ctx.set_lambda(None)
-            ctx.run(params)
-        results = []
-        t0 = time.perf_counter()
-        for _ in range(a.steps):
-            ctx.set_lambda(None)
-            results.append(ctx.run(params))
-        wall = time.perf_counter() - t0
+        shape = ctx.set_city(ncx, ncy, 1.0, dz, blds, (1, u_ref, 10.0, 0.5, 270.0))
+        wall, results = reference_schedule(ctx, params, a.steps, a.warmup)
         div_ref = ctx.divergence_linf(False)
-        # velocity gradient tensor of the corrected field (QES-Turb derivative step)
-        grad_ms = min(ctx.velocity_gradient(uniform_z=False, copy=False)[1] for _ in range(3))
-
+        div_phys = ctx.divergence_linf(True)
+        if "cpu" in legs:
+            cpu = cpu_baseline_resident(ctx, "1000x1000x150 city, wind 270 deg")
+    lups_it = shape.interior_lups()
     its = results[-1]["iterations"]
-    lups = city.interior_lups() * its
-    glups = lups * a.steps / wall / 1e9
-    ms_step = wall / a.steps * 1e3
-    ms_sor = float(np.mean([r["ms_sor"] for r in results]))
     last = results[-1]
-    fused = last["path"] == Q.PATH_FUSED
-    launches = its if fused else 2 * its
-    launch_ms = ms_sor / launches
-    bpl = bytes_per_lup(last)
-    launch_bytes = bpl * city.interior_lups() / (1 if fused else 2)
-    achieved = launch_bytes / (launch_ms * 1e-3) / 1e9
-    if last["path"] == Q.PATH_PLAIN:
-        kernel = "k_rb_plain"
-    elif fused:
-        kernel = "k_sor_fused"
-    else:
-        kernel = "k_sor_split<%s, %d>" % ("true" if last["compressed"] else "false", last["tile_v"])
+    roof, ms_sor = roofline_of(results, lups_it, "city1000x1000x150")
+    glups = lups_it * its * a.steps / wall / 1e9
     out = {
         "metric": METRIC,
         "value": round(glups, 3),
@@ -609,42 +784,43 @@ def main():
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(ms_step, 3),
+        "ms_per_step": round(wall / a.steps * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded city generator, inputs resident in HBM)",
-        "config": {"workload": desc, "cells": city.ncell, "updates_per_iteration": city.interior_lups(),
-                   "iterations": its, "omega": 1.78, "path": results[-1]["path"],
+        "config": {"workload": desc, "cells": shape.ncell, "updates_per_iteration": lups_it,
+                   "iterations": its, "omega": 1.78, "path": last["path"],
                    "wall_mask_stencil": bool(last["compressed"]),
                    "tile": [last["tile_k"], last["tile_v"], last["wg_per_cu"]]},
-        "t_ref_ms": round(ms_step, 3),
+        "t_ref_ms": round(wall / a.steps * 1e3, 3),
         "sor_ms": round(ms_sor, 3),
         "div_linf_ref_operator": div_ref,
-        "velocity_gradient": {"ms": round(grad_ms, 4), "bytes_per_cell": 52,
-                              "achieved_gbs": round(52.0 * city.ncell / (grad_ms * 1e-3) / 1e9, 1)},
-        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": load_traffic(kernel), "bytes_per_lup": bpl,
-                     "avg_launch_us": round(launch_ms * 1e3, 2)},
-        "cpu_baseline": None,
+        "div_linf_physical": div_phys,
+        "roofline": roof,
+        "cpu_baseline": cpu,
     }
-    fp64_ref = None
-    seeds = list(range(1_000_000, 1_000_000 + a.ttol_cities))
-    if a.cpu_budget > 0:
-        out["cpu_baseline"] = cpu_baseline(city, a.cpu_budget)
-        if seeds:
-            from qesmc import warmstart as W
-
-            fp64_ref = fp64_reference_fields({s_: W.seeded_city(s_) for s_ in seeds})
-    if seeds:
-        out["time_to_tol"] = time_to_tol_leg(seeds, fp64_ref)
-        if fp64_ref is not None:
-            out["time_to_tol"]["fp64_oracle_iterations"] = [fp64_ref[s_][3] for s_ in seeds]
-    if a.batch > 0:
-        out["batch"] = batch_leg(a.batch)
+    print(f"headline: {out['value']} GLUPS, {out['ms_per_step']} ms per solve", file=sys.stderr, flush=True)
+    run_legs(a, legs, out, path, tiles, (lups_it / (cpu["value"] * 1e6)) if cpu else None)
     print(json.dumps(out))
+
+
+def run_legs(a, legs, out, path, tiles, cpu_ms_per_it):
+    """The extra legs named in `legs`, each into its key of `out`."""
+    if "city1000_tol" in legs:
+        out["city1000_tol"] = c5_tol_leg(cpu_ms_per_it=cpu_ms_per_it)
+    if "city400" in legs:
+        out["city400"] = city400_leg(path, tiles)
+    if "ttol" in legs and a.ttol_cities > 0:
+        from qesmc import warmstart as W
+
+        seeds = list(range(1_000_000, 1_000_000 + a.ttol_cities))
+        fp64_ref = fp64_reference_fields({s_: W.seeded_city(s_) for s_ in seeds})
+        out["time_to_tol"] = time_to_tol_leg(seeds, fp64_ref)
+        out["time_to_tol"]["fp64_oracle_iterations"] = [fp64_ref[s_][3] for s_ in seeds]
+    if "batch" in legs and a.batch > 0:
+        out["batch"] = batch_leg(a.batch)
 
 
 if __name__ == "__main__":

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define QESMC_ABI_VERSION 1
+#define QESMC_ABI_VERSION 2
 
 /* ---- status codes ---------------------------------------------------------
  * The reference exits the process on every error (Solver_GPU_GlobalMemory.cu:
@@ -92,7 +92,16 @@ typedef struct qesmc_params {
                        choice)                                                */
   int tile_v;       /* split path: half-cells per lane, 1, 2 or 4 (0 = choice) */
   int wg_per_cu;    /* split path: resident work-groups per CU (0 = choice)    */
-  /* The tile fields only change speed, never results. */
+  int slab_planes;  /* split path: k-slab of the interleaved red/black schedule
+                       (0 = library's choice, -1 = two whole-domain passes)   */
+  /* The tile fields and slab_planes only change speed, never results. */
+  float stop_extend; /* stop rule.  0: the reference's (iterate while
+                        max|dlambda| > tol, Solver_CPU_RB.cpp:95).  > 0: the
+                        production rule for a field accuracy: the first
+                        iteration t* that the reference's rule would not run
+                        fixes the end at t* + ceil(stop_extend * t*); the
+                        iterations up to it run whatever max|dlambda| does.
+                        Decided by the device gate, like the reference's.   */
 } qesmc_params;
 
 /* qesmc_params.flags: rebuild the stencil encoding from the resident
@@ -123,7 +132,8 @@ int qesmc_create(qesmc_ctx **out, int device);
 int qesmc_destroy(qesmc_ctx *ctx);
 
 /* Reference defaults: omega 1.78f, tol 1e-9f, itermax 500, alpha 1,
- * path AUTO, check_every 64, flags 0, tile_k 0. */
+ * path AUTO, check_every 64, flags 0, tile fields 0, slab_planes 0,
+ * stop_extend 0 (the reference's stop rule). */
 void qesmc_default_params(qesmc_params *p);
 
 const char *qesmc_status_string(int status);
@@ -248,6 +258,20 @@ int qesmc_get_inputs(qesmc_ctx *ctx, float *e, float *f, float *g, float *h, flo
  * on the resident dz_array.  ms (optional): device time of the kernel. */
 int qesmc_velocity_gradient(qesmc_ctx *ctx, int uniform_z, float dz, float *G, double *ms);
 
+/* Wall::wallLogBC (Wall.cpp:352-797) on the resident corrected field, as
+ * qesWindsMain.cpp:157-165 applies it after a solve when logLawFlag == 1
+ * (u0 = u; wallLogBC; u = u0): for every fluid cell next to a solid (flag 0
+ * or 2) below, above, behind (i > 0), in front, right (j > 0) or left, in
+ * that pass order (the lists of Wall.cpp:288-340), the face (i,j,k) gets a
+ * zero wall-normal component and the 20-step log-law speed of the next cell
+ * out along the wall-parallel direction of that cell.  z0: WGD->z0 (the
+ * sensor's site z0, WINDSGeneralData.cpp:396-406); dz is the smallest
+ * dz_array entry and dy = dx, as the reference's Domain reports them.  Cut
+ * cells (flags 7/8, Wall.cpp:381-655) are not built: QESMC_ERR_UNSUPPORTED
+ * and the field is left unchanged.  ms (optional): device time of the six
+ * passes. */
+int qesmc_wall_log_bc(qesmc_ctx *ctx, float z0, double *ms);
+
 /* ---- ensembles on one geometry (no counterpart in the reference) ----------
  *
  * The discrete problem is linear in (u0, v0, w0), and a single-sensor inflow
@@ -262,6 +286,15 @@ int qesmc_lambda_combine(qesmc_ctx *ctx, int n, qesmc_ctx *const *srcs, const fl
 /* *out = max over all faces of |u_a - u_b|, |v_a - v_b|, |w_a - w_b| (FP64;
  * a NaN difference gives +inf).  Contexts on one device, same grid. */
 int qesmc_velocity_linf_diff(qesmc_ctx *a, qesmc_ctx *b, double *out);
+
+/* Velocity probes: the resident corrected u, v, w at n face indices
+ * (i + j*nx + k*nx*ny, each < nx*ny*nz), gathered on the device, so a field
+ * can be compared with sampled reference values without copying it whole.
+ * The index list stays in the context for later qesmc_get_probes calls
+ * (until the grid changes).  No counterpart in the reference. */
+int qesmc_set_probes(qesmc_ctx *ctx, long long n, const long long *faces);
+/* u, v, w: n_probes floats each, in the order of the index list. */
+int qesmc_get_probes(qesmc_ctx *ctx, float *u, float *v, float *w);
 
 /* ---- QES workspace files (NetCDF classic, 64-bit offset) ------------------ */
 

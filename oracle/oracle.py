@@ -61,6 +61,8 @@ def lib():
     L.oracle_extrude_profile.argtypes = [I, I, I, f32p, f32p, f32p, f32p, f32p]
     L.oracle_set_velocity_zero.argtypes = [I, I, I, i32p, f32p, f32p, f32p]
     L.oracle_velocity_gradient.argtypes = [I, I, I, Fl, Fl, Fl, f32p, I, i32p, f32p, f32p, f32p, f32p]
+    L.oracle_wall_log_bc.argtypes = [I, I, I, Fl, Fl, Fl, i32p, f32p, f32p, f32p]
+    L.oracle_wall_log_bc.restype = C.c_long
     L.oracle_num_threads.restype = I
     L.oracle_set_num_threads.argtypes = [I]
     _lib = L
@@ -263,6 +265,19 @@ def velocity_gradient(p: Problem, u, v, w, uniform_z=False, dz=None):
                                    p.icellflag, np.ascontiguousarray(u, np.float32),
                                    np.ascontiguousarray(v, np.float32), np.ascontiguousarray(w, np.float32), G)
     return G
+
+
+def wall_log_bc(p: Problem, u, v, w, z0, dz=None):
+    """Wall::wallLogBC on a solved stair-step field (Wall.cpp:659-797, applied
+    as qesWindsMain.cpp:157-165 does: u0 = u, wallLogBC, u = u0).  Returns
+    (u, v, w, number of wall faces); dz defaults to the Domain's dz, the
+    smallest dz_array entry (Domain.cpp:89,131)."""
+    u, v, w = (np.array(a, np.float32, copy=True) for a in (u, v, w))
+    dzv = float(np.min(p.dz_array)) if dz is None else float(dz)
+    n = lib().oracle_wall_log_bc(p.nx, p.ny, p.nz, p.dx, dzv, float(z0), p.icellflag, u, v, w)
+    if n < 0:
+        raise ValueError("cut-cell flags (7/8) present: only the stair-step passes are restated")
+    return u, v, w, int(n)
 
 
 def set_num_threads(t: int):

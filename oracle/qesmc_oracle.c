@@ -721,6 +721,101 @@ void oracle_velocity_gradient(int nx, int ny, int nz, float dx, float dy, float 
   free(z);
 }
 
+/* ------------------------------------------------------------------------ */
+/* f4: Wall::wallLogBC on stair-step walls                                  */
+/* ------------------------------------------------------------------------ */
+
+/* One wall-parallel log-law correction (the body of every loop of
+ * Wall.cpp:677-797): 20 fixed-point steps of the friction velocity from the
+ * second cell's velocity (a, b: its two wall-parallel components at face
+ * `nb`), writing the normal component (*pn = 0) and the parallel ones of face
+ * `f`.  Math as the reference evaluates it: Wall.h includes <math.h> with
+ * `using namespace std` (Wall.h:38,46), so log/atan2/cos/sin of float
+ * arguments are the float overloads, pow(float, 2.0) is double, and the
+ * magnitude is a double sqrt stored to float.  vk = 0.4f
+ * (WINDSGeneralData.h:161). */
+static void log_law_face(float *A, float *B, float *N, long f, long nb, float dist1, float dist2, float z0)
+{
+  const float vk = 0.4f;
+  float ustar_wall = 0.1f;
+  for (int iter = 0; iter < 20; iter++) {
+    float wind_dir = atan2f(B[nb], A[nb]);
+    float vel_mag2 = (float)sqrt(pow((double)A[nb], 2.0) + pow((double)B[nb], 2.0));
+    float vel_mag1 = vel_mag2 - (ustar_wall / vk) * logf(dist2 / dist1);
+    N[f] = 0;
+    A[f] = vel_mag1 * cosf(wind_dir);
+    B[f] = vel_mag1 * sinf(wind_dir);
+    float new_ustar = vk * vel_mag1 / logf(dist1 / z0);
+    ustar_wall = new_ustar;
+  }
+}
+
+/* Wall::wallLogBC (Wall.cpp:352-797) for stair-step geometry, applied the way
+ * qesWindsMain.cpp:157-165 applies it to a solved field: u0 = u; wallLogBC;
+ * u = u0 (so u, v, w are updated in place here).  The cut-cell loop over
+ * wall_indices (Wall.cpp:381-655) visits only flags 7/8, which stair-step
+ * geometry does not have; returns -1 (nothing done) when a 7/8 flag exists.
+ *
+ * The six index lists are those of Wall::defineWalls (Wall.cpp:288-340): for
+ * k in [1,nz-3], i in [0,nx-2], j in [0,ny-2] (i outer, j inner), every cell
+ * with icellflag not in {0,2} (and not 8) whose neighbour below / above / back
+ * (i > 0) / front / right (j > 0) / left is solid (flag 0 or 2) contributes
+ * its face index face(i,j,k); neighbours are read with cellAdd, so front and
+ * left wrap at i = nx-2 / j = ny-2 (quirk A11), and every fluid cell at k = 1
+ * has the ground ghost layer below it.  The passes run in the reference's
+ * order -- below, above (dist 0.5 dz / 1.5 dz), back, front (0.5 dx / 1.5 dx),
+ * right, left (0.5 dy / 1.5 dy, with dy = dx: getDomainSize, Domain.h:104)
+ * -- each reading its second-cell face at +nx*ny, -nx*ny, +1, -1, +nx, -nx,
+ * so a later pass sees the faces an earlier one rewrote.  dz is the Domain's
+ * dz, the smallest dz_array entry for a stretched grid (Domain.cpp:89,131).
+ * z0 is WGD->z0 (WINDSGeneralData.cpp:396-406: the sensor's site_z0).
+ * Returns the number of wall faces processed. */
+long oracle_wall_log_bc(int nx, int ny, int nz, float dx, float dz, float z0, const int *icellflag, float *u,
+                        float *v, float *w)
+{
+  const long sy = nx - 1, sz = (long)(nx - 1) * (ny - 1);
+  const long ncell = sz * (nz - 1);
+  for (long c = 0; c < ncell; ++c)
+    if (icellflag[c] == 7 || icellflag[c] == 8) return -1;
+  long cap = ncell, cnt[6] = {0, 0, 0, 0, 0, 0};
+  long *list[6];
+  for (int p = 0; p < 6; ++p) list[p] = (long *)malloc(sizeof(long) * (size_t)(cap > 0 ? cap : 1));
+  for (int k = 1; k < nz - 2; k++)
+    for (int i = 0; i < nx - 1; i++)
+      for (int j = 0; j < ny - 1; j++) {
+        long c = CELL(i, j, k);
+        long fc = FACE(i, j, k);
+        int fl = icellflag[c];
+        if (fl != 0 && fl != 2 && fl != 8) {
+          if (solid(icellflag[c - sz])) list[0][cnt[0]++] = fc;
+          if (solid(icellflag[c + sz])) list[1][cnt[1]++] = fc;
+          if (solid(icellflag[c - 1]) && i > 0) list[2][cnt[2]++] = fc;
+          if (solid(icellflag[c + 1])) list[3][cnt[3]++] = fc;
+          if (solid(icellflag[c - sy]) && j > 0) list[4][cnt[4]++] = fc;
+          if (solid(icellflag[c + sy])) list[5][cnt[5]++] = fc;
+        }
+      }
+  const long fz = (long)nx * ny;
+  const float dy = dx;
+  float d1 = 0.5f * dz, d2 = 1.5f * dz;
+  for (long q = 0; q < cnt[0]; ++q) log_law_face(u, v, w, list[0][q], list[0][q] + fz, d1, d2, z0);
+  for (long q = 0; q < cnt[1]; ++q) log_law_face(u, v, w, list[1][q], list[1][q] - fz, d1, d2, z0);
+  d1 = 0.5f * dx;
+  d2 = 1.5f * dx;
+  for (long q = 0; q < cnt[2]; ++q) log_law_face(v, w, u, list[2][q], list[2][q] + 1, d1, d2, z0);
+  for (long q = 0; q < cnt[3]; ++q) log_law_face(v, w, u, list[3][q], list[3][q] - 1, d1, d2, z0);
+  d1 = 0.5f * dy;
+  d2 = 1.5f * dy;
+  for (long q = 0; q < cnt[4]; ++q) log_law_face(u, w, v, list[4][q], list[4][q] + nx, d1, d2, z0);
+  for (long q = 0; q < cnt[5]; ++q) log_law_face(u, w, v, list[5][q], list[5][q] - nx, d1, d2, z0);
+  long total = 0;
+  for (int p = 0; p < 6; ++p) {
+    total += cnt[p];
+    free(list[p]);
+  }
+  return total;
+}
+
 int oracle_num_threads(void)
 {
 #ifdef _OPENMP

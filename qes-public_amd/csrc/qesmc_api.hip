@@ -28,6 +28,8 @@ hipError_t qesmc_launch_iteration(qesmc_ctx *x, int path, bool masked, int t, in
 hipError_t qesmc_launch_finish(qesmc_ctx *x, bool mirror, int alpha1, int alpha2);
 hipError_t qesmc_launch_div_linf(qesmc_ctx *x, int physical);
 hipError_t qesmc_launch_set_u32(qesmc_ctx *x, unsigned *p, unsigned v);
+hipError_t qesmc_launch_fill_u32(qesmc_ctx *x, unsigned *p, long long count, unsigned v);
+hipError_t qesmc_launch_rows_summary(qesmc_ctx *x, int launched, float tol);
 
 namespace {
 
@@ -222,6 +224,8 @@ void qesmc_default_params(qesmc_params *p)
   p->tile_k = 0;
   p->tile_v = 0;
   p->wg_per_cu = 0;
+  p->slab_planes = 0;
+  p->stop_extend = 0.f;
 }
 
 int qesmc_create(qesmc_ctx **out, int device)
@@ -249,7 +253,7 @@ int qesmc_create(qesmc_ctx **out, int device)
       x->n_cu = ncu;
   }
   if (fail(hipHostMalloc((void **)&x->h_poll, (2 * qesmc::kErrSlots + 8) * sizeof(unsigned), hipHostMallocDefault))) goto bad;
-  if (fail(dalloc(&x->d_status, 1))) goto bad;
+  if (fail(dalloc(&x->d_status, qesmc::kStatusWords))) goto bad;
   if (fail(dalloc(&x->d_norm, 1))) goto bad;
   *out = x;
   return QESMC_OK;
@@ -268,6 +272,8 @@ int qesmc_destroy(qesmc_ctx *x)
   dfree(x->d_status);
   dfree(x->d_norm);
   dfree(x->d_cur);
+  dfree(x->d_probe);
+  dfree(x->d_probe_val);
   if (x->h_poll) (void)hipHostFree(x->h_poll);
   for (auto &ev : x->ev)
     if (ev) (void)hipEventDestroy(ev);
@@ -353,7 +359,8 @@ int run_prepare(qesmc_ctx *x, const qesmc_params *pp, RunState &st)
   if (p.path != QESMC_PATH_AUTO && p.path != QESMC_PATH_PLAIN && p.path != QESMC_PATH_SPLIT &&
       p.path != QESMC_PATH_FUSED)
     return QESMC_ERR_ARG;
-  if (p.tile_k < 0 || p.wg_per_cu < 0) return QESMC_ERR_ARG;
+  if (p.tile_k < 0 || p.wg_per_cu < 0 || p.slab_planes < -1) return QESMC_ERR_ARG;
+  if (!(p.stop_extend >= 0.f && p.stop_extend <= 1000.f)) return QESMC_ERR_ARG;  // also refuses NaN
   if (p.tile_v != 0 && p.tile_v != 1 && p.tile_v != 2 && p.tile_v != 4) return QESMC_ERR_ARG;
   st.path = (p.path == QESMC_PATH_PLAIN || p.path == QESMC_PATH_FUSED) ? p.path : QESMC_PATH_SPLIT;
   HIPCHK(hipSetDevice(x->device));
@@ -399,6 +406,8 @@ int run_prepare(qesmc_ctx *x, const qesmc_params *pp, RunState &st)
   // lane, 5 for one (fewer, longer-lived work-groups beat full occupancy on
   // every measured size but the HBM-bound 1000x1000x150, where they lose 1 %)
   x->wg_per_cu = p.wg_per_cu > 0 ? p.wg_per_cu : x->wg_per_cu_auto > 0 ? x->wg_per_cu_auto : x->tile_v >= 2 ? 4 : 5;
+  // k-slab schedule of the split path (results never depend on it)
+  x->slab_planes = p.slab_planes > 0 ? p.slab_planes : p.slab_planes < 0 ? 0 : x->slab_planes_auto;
   if (st.path == QESMC_PATH_PLAIN) {
     // the classic layout's own default: 8 planes per work-group (its
     // measured best; the split path's choice above does not apply to it)
@@ -425,8 +434,16 @@ int run_prepare(qesmc_ctx *x, const qesmc_params *pp, RunState &st)
   }
   hipStream_t s = x->stream;
   HIPCHK(hipEventRecord(x->ev[0], s));
-  HIPCHK(hipMemsetAsync(x->d_status, 0, sizeof(unsigned), s));
-  HIPCHK(hipMemsetAsync(x->d_err, 0, sizeof(unsigned) * (size_t)(rows * kErrSlots), s));
+  HIPCHK(qesmc_launch_fill_u32(x, x->d_status, 1, 0u));
+  // the stop rule: no end recorded yet, and the extension factor (0 for the
+  // reference's rule) that every kernel's gate reads
+  {
+    unsigned ext;
+    std::memcpy(&ext, &p.stop_extend, 4);
+    HIPCHK(qesmc_launch_fill_u32(x, x->d_status + kStopAt, 1, 0xffffffffu));
+    HIPCHK(qesmc_launch_fill_u32(x, x->d_status + kStopExtend, 1, ext));
+  }
+  HIPCHK(qesmc_launch_fill_u32(x, x->d_err, rows * kErrSlots, 0u));
   HIPCHK(qesmc_launch_set_u32(x, x->d_err, 0x3f800000u));  // row 0: 1.0f
   if (st.path != QESMC_PATH_PLAIN) HIPCHK(qesmc_launch_pack(x, true));
   if (st.path == QESMC_PATH_FUSED) {
@@ -453,17 +470,17 @@ int run_finish(qesmc_ctx *x, const RunState &st, int launched, qesmc_result *res
   if (st.path == QESMC_PATH_FUSED) HIPCHK(qesmc_launch_unpack_selected(x));
   HIPCHK(qesmc_launch_finish(x, st.first_runs, p.alpha1, p.alpha2));
   HIPCHK(hipEventRecord(x->ev[3], s));
+  // iteration t ran iff max(row t) > tol: the count and the last residual are
+  // reduced on the device, so only four words come back
+  HIPCHK(qesmc_launch_rows_summary(x, launched, p.tol));
+  unsigned *sw = x->h_poll + 2 * kErrSlots;
+  HIPCHK(hipMemcpyAsync(sw, x->d_status, 4 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   x->have_velocity = true;
-
-  std::vector<unsigned> herr(((size_t)launched + 1) * kErrSlots);
-  HIPCHK(hipMemcpy(herr.data(), x->d_err, sizeof(unsigned) * herr.size(), hipMemcpyDeviceToHost));
-  int count = 0;
-  while (count < launched && row_max(&herr[(size_t)count * kErrSlots]) > p.tol) ++count;
-  const float last = row_max(&herr[(size_t)count * kErrSlots]);
-  unsigned stw = 0;
-  HIPCHK(hipMemcpy(&stw, x->d_status, sizeof(unsigned), hipMemcpyDeviceToHost));
-  const int rc = (stw & ST_NONFINITE) ? QESMC_ERR_NONFINITE : QESMC_OK;
+  const int count = (int)sw[2];
+  float last;
+  std::memcpy(&last, sw + 3, 4);
+  const int rc = (sw[0] & ST_NONFINITE) ? QESMC_ERR_NONFINITE : QESMC_OK;
   if (res) {
     float ms_total = 0.f, ms_sor = 0.f;
     (void)hipEventElapsedTime(&ms_total, x->ev[0], x->ev[3]);
@@ -489,7 +506,15 @@ int run_finish(qesmc_ctx *x, const RunState &st, int launched, qesmc_result *res
 
 extern "C" {
 
-int qesmc_run(qesmc_ctx *x, const qesmc_params *pp, qesmc_result *res)
+int qesmc_run(qesmc_ctx *x, const qesmc_params *pp, qesmc_result *res) { return qesmc::run_solve(x, pp, res); }
+
+}  // extern "C"
+
+namespace qesmc {
+
+// One resident solve: preparation, the iteration loop with convergence
+// polling one chunk behind the device, and the finish (qesmc_run).
+int run_solve(qesmc_ctx *x, const qesmc_params *pp, qesmc_result *res)
 {
   qesmc::RunState st;
   int rc = qesmc::run_prepare(x, pp, st);
@@ -517,12 +542,21 @@ int qesmc_run(qesmc_ctx *x, const qesmc_params *pp, qesmc_result *res)
         if (chunk >= 2 && (e = hipEventSynchronize(x->ev[4 + slot])) != hipSuccess) { rc = map_hip(e); break; }
         e = hipMemcpyAsync(dst, x->d_err + (long long)(t + 1) * qesmc::kErrSlots,
                            sizeof(unsigned) * qesmc::kErrSlots, hipMemcpyDeviceToHost, s);
+        // the production rule's recorded end (all ones until fixed)
+        unsigned *end_h = x->h_poll + 2 * qesmc::kErrSlots + 4 + slot;
+        if (e == hipSuccess)
+          e = hipMemcpyAsync(end_h, x->d_status + qesmc::kStopAt, sizeof(unsigned), hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipEventRecord(x->ev[4 + slot], s);
         if (e != hipSuccess) { rc = map_hip(e); break; }
         if (chunk >= 1) {
           const int prev = slot ^ 1;
           if ((e = hipEventSynchronize(x->ev[4 + prev])) != hipSuccess) { rc = map_hip(e); break; }
-          if (!(qesmc::row_max(x->h_poll + prev * qesmc::kErrSlots) > p.tol)) stop = true;
+          if (p.stop_extend > 0.f) {
+            // every iteration from the recorded end on is a no-op
+            if (x->h_poll[2 * qesmc::kErrSlots + 4 + prev] <= (unsigned)(t + 1)) stop = true;
+          } else if (!(qesmc::row_max(x->h_poll + prev * qesmc::kErrSlots) > p.tol)) {
+            stop = true;
+          }
         }
         ++chunk;
       }
@@ -534,6 +568,10 @@ int qesmc_run(qesmc_ctx *x, const qesmc_params *pp, qesmc_result *res)
   }
   return qesmc::run_finish(x, st, launched, res);
 }
+
+}  // namespace qesmc
+
+extern "C" {
 
 int qesmc_get_velocity(qesmc_ctx *x, float *u, float *v, float *w)
 {

@@ -114,7 +114,7 @@ __global__ __launch_bounds__(256) void k_sor_batch(const BatchItem *__restrict__
       const BatchItem &it = items[cur];
       cbeg = it.tile_begin;
       cend = cur + 1 < n ? items[cur + 1].tile_begin : total;
-      open = wave_max_b(__uint_as_float(it.err[(long long)t * kErrSlots + lane])) > tol;
+      open = gate_pass(wave_max_b(__uint_as_float(it.err[(long long)t * kErrSlots + lane])), t, tol, it.status);
       d = it.d;
       kb = it.kb;
       tpp = it.tiles_per_plane;
@@ -192,7 +192,14 @@ __global__ __launch_bounds__(64) void k_count_active(const BatchItem *items, int
     for (int i = threadIdx.x; i < n; i += 64) {
       float mx = 0.f;
       for (int q = 0; q < kErrSlots; ++q) mx = fmaxf(mx, __uint_as_float(items[i].err[(long long)t * kErrSlots + q]));
-      c += mx > tol ? 1 : 0;
+      // the gate of iteration t (gate_pass, without recording the end)
+      const unsigned *st = items[i].status;
+      const float e = __uint_as_float(st[kStopExtend]);
+      bool open;
+      if (!(e > 0.f)) open = mx > tol;
+      else if ((unsigned)t >= *(volatile const unsigned *)(st + kStopAt)) open = false;
+      else open = mx > tol || (unsigned)t + (unsigned)ceilf(e * (float)t) > (unsigned)t;
+      c += open ? 1 : 0;
     }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
@@ -275,9 +282,11 @@ int qesmc_run_batch(qesmc_ctx **ctxs, int n, const qesmc_params *params, qesmc_r
   BCHK(hipSetDevice(lead->device));
   hipStream_t bs = lead->stream;
   std::vector<hipStream_t> own_streams(n);
+  // every instance's own work completes before any context is moved to the
+  // batch stream, so a failure here leaves every context untouched
+  for (int i = 0; i < n; ++i) BCHK(hipStreamSynchronize(ctxs[i]->stream));
   for (int i = 0; i < n; ++i) {
     own_streams[i] = ctxs[i]->stream;
-    BCHK(hipStreamSynchronize(own_streams[i]));
     ctxs[i]->stream = bs;  // every phase of every instance on the batch stream
   }
   auto restore = [&]() {
@@ -448,7 +457,14 @@ int qesmc_autotune(qesmc_ctx *x, const qesmc_params *base, int iterations, qesmc
   BCHK(hipSetDevice(x->device));
   float *saved = nullptr;
   BCHK(hipMalloc((void **)&saved, sizeof(float) * (size_t)x->ncell));
-  BCHK(hipMemcpyAsync(saved, x->d_lam, sizeof(float) * (size_t)x->ncell, hipMemcpyDeviceToDevice, x->stream));
+  {
+    const hipError_t e =
+        hipMemcpyAsync(saved, x->d_lam, sizeof(float) * (size_t)x->ncell, hipMemcpyDeviceToDevice, x->stream);
+    if (e != hipSuccess) {
+      (void)hipFree(saved);
+      return map_hip_error(e);
+    }
+  }
   static const int cand[][3] = {{4, 2, 4}, {8, 2, 4}, {10, 2, 6}, {6, 2, 4}, {7, 2, 5},
                                 {3, 2, 4}, {10, 2, 8}, {6, 2, 8}, {2, 2, 4}, {4, 1, 5},
                                 {6, 1, 6}, {10, 1, 6}, {2, 1, 5}, {1, 2, 4}, {1, 1, 6}};
@@ -466,8 +482,9 @@ int qesmc_autotune(qesmc_ctx *x, const qesmc_params *base, int iterations, qesmc
     qesmc_result r;
     double ms = 0.0;
     for (int rep = 0; rep < 2 && rc == QESMC_OK; ++rep) {  // first repetition warms caches
-      BCHK(hipMemcpyAsync(x->d_lam, saved, sizeof(float) * (size_t)x->ncell, hipMemcpyDeviceToDevice, x->stream));
-      rc = qesmc_run(x, &p, &r);
+      const hipError_t e =
+          hipMemcpyAsync(x->d_lam, saved, sizeof(float) * (size_t)x->ncell, hipMemcpyDeviceToDevice, x->stream);
+      rc = e != hipSuccess ? map_hip_error(e) : qesmc_run(x, &p, &r);
       ms = r.ms_sor;
     }
     if (rc != QESMC_OK) break;

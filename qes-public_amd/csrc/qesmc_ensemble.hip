@@ -65,6 +65,20 @@ __global__ __launch_bounds__(256) void k_vel_diff(const float *__restrict__ a0, 
   }
 }
 
+// out[q] = u[idx[q]], out[n + q] = v[idx[q]], out[2n + q] = w[idx[q]].
+__global__ __launch_bounds__(256) void k_probe(const float *__restrict__ u, const float *__restrict__ v,
+                                               const float *__restrict__ w, const long long *__restrict__ idx,
+                                               long long n, float *__restrict__ out)
+{
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long q = blockIdx.x * 256LL + threadIdx.x; q < n; q += stride) {
+    const long long f = idx[q];
+    out[q] = u[f];
+    out[n + q] = v[f];
+    out[2 * n + q] = w[f];
+  }
+}
+
 bool same_grid(const qesmc_ctx *a, const qesmc_ctx *b)
 {
   return a->d.nx == b->d.nx && a->d.ny == b->d.ny && a->d.nz == b->d.nz;
@@ -113,6 +127,52 @@ int qesmc_velocity_linf_diff(qesmc_ctx *a, qesmc_ctx *b, double *out)
   ECHK(hipGetLastError());
   ECHK(hipMemcpyAsync(out, a->d_norm, sizeof(double), hipMemcpyDeviceToHost, a->stream));
   ECHK(hipStreamSynchronize(a->stream));
+  return QESMC_OK;
+}
+
+int qesmc_set_probes(qesmc_ctx *x, long long n, const long long *faces)
+{
+  if (!x || n < 0 || (n > 0 && !faces)) return QESMC_ERR_ARG;
+  if (!x->d_u) return QESMC_ERR_STATE;
+  for (long long q = 0; q < n; ++q)
+    if (faces[q] < 0 || faces[q] >= x->nface) return QESMC_ERR_ARG;
+  ECHK(hipSetDevice(x->device));
+  if (n > x->probe_cap) {
+    ECHK(hipStreamSynchronize(x->stream));
+    if (x->d_probe) (void)hipFree(x->d_probe);
+    if (x->d_probe_val) (void)hipFree(x->d_probe_val);
+    x->d_probe = nullptr;
+    x->d_probe_val = nullptr;
+    x->probe_cap = x->n_probe = 0;
+    ECHK(hipMalloc((void **)&x->d_probe, sizeof(long long) * (size_t)n));
+    ECHK(hipMalloc((void **)&x->d_probe_val, 3 * sizeof(float) * (size_t)n));
+    x->probe_cap = n;
+  }
+  if (n > 0)
+    ECHK(hipMemcpyAsync(x->d_probe, faces, sizeof(long long) * (size_t)n, hipMemcpyHostToDevice, x->stream));
+  ECHK(hipStreamSynchronize(x->stream));
+  x->n_probe = n;
+  x->probe_nface = x->nface;
+  return QESMC_OK;
+}
+
+int qesmc_get_probes(qesmc_ctx *x, float *u, float *v, float *w)
+{
+  if (!x) return QESMC_ERR_ARG;
+  if (x->n_probe > 0 && (!u || !v || !w)) return QESMC_ERR_ARG;
+  if (!x->d_u || x->probe_nface != x->nface) return QESMC_ERR_STATE;
+  if (x->n_probe == 0) return QESMC_OK;
+  ECHK(hipSetDevice(x->device));
+  const long long n = x->n_probe;
+  long long blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_probe, dim3((unsigned)blocks), dim3(256), 0, x->stream, x->d_u, x->d_v, x->d_w, x->d_probe, n,
+                     x->d_probe_val);
+  ECHK(hipGetLastError());
+  float *dst[3] = {u, v, w};
+  for (int a = 0; a < 3; ++a)
+    ECHK(hipMemcpyAsync(dst[a], x->d_probe_val + a * n, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, x->stream));
+  ECHK(hipStreamSynchronize(x->stream));
   return QESMC_OK;
 }
 

@@ -38,6 +38,36 @@ enum : unsigned {
   ST_ZERO_DIAG = 4u,      // an updated cell has e+f+g+h+m+n == 0
 };
 
+// Words of ctx->d_status: [0] status bits, [1..3] convergence-row summary
+// of the last run, [4] the production stop rule's end iteration (all ones
+// until fixed), [5] its extension factor (float bits, 0 = reference rule).
+constexpr int kStatusWords = 8;
+constexpr int kStopAt = 4;
+constexpr int kStopExtend = 5;
+
+// Iteration gate, decided identically by every wave of a launch.  g is the
+// max of convergence row t (the previous iteration's max|dlambda|, 1.0 for
+// t = 0).  Reference rule (extension 0): iteration t runs iff g > tol
+// (Solver_CPU_RB.cpp:95).  Production rule (extension e > 0): the first
+// iteration t* whose row is not above tol fixes the end at
+// t* + ceil(e * t*), recorded in status[kStopAt] with atomicMin (every wave
+// that sees t* proposes the same value, and later launches read it);
+// iterations before the end run whatever their row says.  Within the launch
+// of t* the recorded end is either unset or that value, both above t*, so
+// the decision cannot differ between waves.
+__device__ inline bool gate_pass(float g, int t, float tol, unsigned *status)
+{
+  const float e = __uint_as_float(status[kStopExtend]);
+  if (!(e > 0.f)) return g > tol;
+  const unsigned tu = (unsigned)t;
+  if (tu >= *(volatile unsigned *)(status + kStopAt)) return false;
+  if (g > tol) return true;
+  const unsigned end = tu + (unsigned)ceilf(e * (float)t);
+  if (end <= tu) return false;
+  if ((threadIdx.x & 63) == 0) atomicMin(status + kStopAt, end);
+  return true;
+}
+
 // max|dlambda| of one iteration is accumulated into kErrSlots striped
 // words (block b -> slot b % kErrSlots): device-scope atomics from every
 // XCD on one address serialise at the memory side, so they are spread.
@@ -81,6 +111,9 @@ struct qesmc_ctx {
   float *d_R = nullptr;
   float *d_lam = nullptr;
   float *d_grad = nullptr;     // velocity gradient tensor, 9 cell arrays (allocated on first use)
+  long long *d_probe = nullptr; // velocity probes: face indices (qesmc_set_probes)
+  float *d_probe_val = nullptr; // gathered u, v, w of the probes (3 x n_probe)
+  long long n_probe = 0, probe_cap = 0, probe_nface = 0;
 
   // colour-split arrays (index = colour)
   float *d_lc[2] = {};
@@ -95,7 +128,7 @@ struct qesmc_ctx {
   // sentinel, row t+1 the striped max|dlambda| of iteration t
   unsigned *d_err = nullptr;
   int err_cap = 0;             // rows allocated
-  unsigned *d_status = nullptr;
+  unsigned *d_status = nullptr; // kStatusWords words, see kStopAt
   double *d_norm = nullptr;
   unsigned *h_poll = nullptr;  // pinned host words: 2 polling rows + status
   int kplanes = 8;             // k planes marched by one SOR wave tile (this run)
@@ -104,6 +137,8 @@ struct qesmc_ctx {
   int tile_v_auto = 0;         // 0: chosen per run from the domain size
   int wg_per_cu = 8;           // resident work-groups per CU of the split kernel
   int wg_per_cu_auto = 0;      // 0: chosen per run from tile_v
+  int slab_planes = 0;         // split path: k-slab of the interleaved red/black schedule (this run; 0: whole passes)
+  int slab_planes_auto = 0;    // default when qesmc_params.slab_planes == 0
   int n_cu = 256;              // compute units of the device
 };
 
@@ -118,6 +153,7 @@ struct RunState {
 };
 int run_prepare(qesmc_ctx *x, const qesmc_params *pp, RunState &st);
 int run_finish(qesmc_ctx *x, const RunState &st, int launched, qesmc_result *res);
+int run_solve(qesmc_ctx *x, const qesmc_params *pp, qesmc_result *res);
 float row_max(const unsigned *row);
 
 // Geometry set-up shared by qesmc_set_geometry and the device city builder:

@@ -47,12 +47,12 @@ __device__ inline float stop_delta(float nv, float ov)
 // Iteration gate: iteration t runs iff max(err row t) > tol, with row 0
 // holding 1.0f, the reference's initial max_error (Solver_CPU_RB.cpp:91,95).
 // Lane l of each wave reads slot l of the row (blockDim.x == 64).
-__device__ inline bool gate_open(const unsigned *err, int t, float tol)
+__device__ inline bool gate_open(const unsigned *err, int t, float tol, unsigned *status)
 {
   float v = __uint_as_float(err[(long long)t * kErrSlots + threadIdx.x]);
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
-  return v > tol;
+  return gate_pass(v, t, tol, status);
 }
 
 // Block's max|dlambda| into its striped slot of row t+1.
@@ -121,7 +121,7 @@ __global__ __launch_bounds__(256) void k_rb_plain(Dims d, int colour, int kplane
                                                   unsigned *err, int t, int itermax,
                                                   unsigned *status)
 {
-  if (t >= itermax || !gate_open(err, t, tol)) return;  // uniform per launch
+  if (t >= itermax || !gate_open(err, t, tol, status)) return;  // uniform per launch
   const int j = blockIdx.y * 4 + threadIdx.y + 1;
   const int tq = blockIdx.x * 64 + threadIdx.x;
   const int k0 = 1 + blockIdx.z * kplanes;
@@ -405,6 +405,51 @@ __global__ __launch_bounds__(256) void k_div_linf(Dims d, double dx, double dy,
 
 __global__ void k_set_u32(unsigned *p, unsigned v) { *p = v; }
 
+// p[0 .. count) = v (a kernel rather than hipMemsetAsync, so that the solve
+// schedule holds no memset nodes if a caller captures it in a graph)
+__global__ __launch_bounds__(256) void k_fill_u32(unsigned *__restrict__ p, long long count, unsigned v)
+{
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long q = blockIdx.x * 256LL + threadIdx.x; q < count; q += stride) p[q] = v;
+}
+
+// The largest entry of one convergence row read as floats, NaN never winning
+// (the host's reading of a row).
+__device__ inline float row_max_dev(const unsigned *row)
+{
+  float mx = 0.f;
+  for (int q = 0; q < kErrSlots; ++q) {
+    const float v = __uint_as_float(row[q]);
+    mx = v > mx ? v : mx;
+  }
+  return mx;
+}
+
+// Iterations that ran: the first row t in [0, launched) whose max is not
+// above tol (iteration t is gated on row t), else launched.  first is
+// pre-set to launched; rows are independent, one thread per row.
+__global__ __launch_bounds__(256) void k_rows_first_stop(const unsigned *__restrict__ rows, int launched, float tol,
+                                                         unsigned *__restrict__ first)
+{
+  for (int t = blockIdx.x * 256 + threadIdx.x; t < launched; t += gridDim.x * 256)
+    if (!(row_max_dev(rows + (long long)t * kErrSlots) > tol)) atomicMin(first, (unsigned)t);
+}
+
+// out[0] = iterations (first[0]; with the production stop rule the end it
+// recorded, if below), out[1] = bits of the max of row out[0]: the last
+// executed iteration's max|dlambda| (the sentinel 1.0 if none ran).
+__global__ void k_rows_summary(const unsigned *__restrict__ rows, const unsigned *__restrict__ first,
+                               const unsigned *__restrict__ status, int launched, unsigned *__restrict__ out)
+{
+  unsigned count = *first;
+  if (__uint_as_float(status[kStopExtend]) > 0.f) {
+    const unsigned end = status[kStopAt];
+    count = end < (unsigned)launched ? end : (unsigned)launched;
+  }
+  out[0] = count;
+  out[1] = __float_as_uint(row_max_dev(rows + (long long)count * kErrSlots));
+}
+
 // ---------------------------------------------------------------------------
 // host-side launch wrappers
 // ---------------------------------------------------------------------------
@@ -536,5 +581,33 @@ hipError_t qesmc_launch_div_linf(qesmc_ctx *x, int physical)
 hipError_t qesmc_launch_set_u32(qesmc_ctx *x, unsigned *p, unsigned v)
 {
   hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, x->stream, p, v);
+  return hipGetLastError();
+}
+
+hipError_t qesmc_launch_fill_u32(qesmc_ctx *x, unsigned *p, long long count, unsigned v)
+{
+  if (count <= 0) return hipSuccess;
+  long long blocks = (count + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, x->stream, p, count, v);
+  return hipGetLastError();
+}
+
+// Iteration count and last max|dlambda| of a run from its convergence rows,
+// into d_status[1] (scan) and d_status[2..3] (count, last bits).
+hipError_t qesmc_launch_rows_summary(qesmc_ctx *x, int launched, float tol)
+{
+  hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, x->stream, x->d_status + 1, (unsigned)launched);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (launched > 0) {
+    int blocks = (launched + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(k_rows_first_stop, dim3(blocks), dim3(256), 0, x->stream, x->d_err, launched, tol,
+                       x->d_status + 1);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_rows_summary, dim3(1), dim3(1), 0, x->stream, x->d_err, x->d_status + 1, x->d_status,
+                     launched, x->d_status + 2);
   return hipGetLastError();
 }

@@ -266,7 +266,7 @@ __global__ __launch_bounds__(256) void k_sor_fused(Dims d, int kb, int qt, int j
   {
     // gate: iteration t runs iff max(err row t) > tol (row 0 = 1.0f)
     const float g = wave_max(__uint_as_float(err[(long long)t * kErrSlots + lane]));
-    if (!(g > tol)) return;  // identical in every wave of the grid
+    if (!gate_pass(g, t, tol, status)) return;  // identical in every wave of the grid
   }
   const long long per = (long long)gridDim.x / kXcds;  // the grid is a multiple of 8
   const long long tile = (long long)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;

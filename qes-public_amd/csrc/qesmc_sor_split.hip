@@ -35,8 +35,9 @@ __device__ inline float wave_max(float v)
 }  // namespace
 
 template <bool MASKED, int VQ>
-__global__ __launch_bounds__(256) void k_sor_split(Dims d, int colour, int kb, int tiles_per_plane,
-                                                   long long ntiles, float *__restrict__ own,
+__global__ __launch_bounds__(256) void k_sor_split(Dims d, int colour, int kb, int k_lo, int k_hi,
+                                                   int tiles_per_plane, long long ntiles,
+                                                   float *__restrict__ own,
                                                    const float *__restrict__ oth,
                                                    const float *__restrict__ Rc,
                                                    const unsigned char *__restrict__ mask,
@@ -57,7 +58,7 @@ __global__ __launch_bounds__(256) void k_sor_split(Dims d, int colour, int kb, i
   {
     // gate: iteration t runs iff max(err row t) > tol (row 0 = 1.0f)
     const float g = wave_max(__uint_as_float(err[(long long)t * kErrSlots + lane]));
-    if (!(g > tol)) return;  // identical in every wave of the grid
+    if (!gate_pass(g, t, tol, status)) return;  // identical in every wave of the grid
   }
   const int xcd = blockIdx.x % kXcds;
   const int local = blockIdx.x / kXcds;
@@ -80,8 +81,8 @@ __global__ __launch_bounds__(256) void k_sor_split(Dims d, int colour, int kb, i
     const int j = jr + 1;
     const int q = p - jr * d.HP;
     if (j > d.NY - 2) continue;
-    const int k0 = 1 + kc * kb;
-    const int k1 = k0 + kb < d.NZ - 1 ? k0 + kb : d.NZ - 1;
+    const int k0 = k_lo + kc * kb;
+    const int k1 = k0 + kb < k_hi ? k0 + kb : k_hi;
     dl = sor_tile<MASKED, VQ>(d, colour, t, j, q, k0, k1, own, oth, Rc, mask, canon, ce, cf, cg, ch, cm, cn, omega,
                               one_m_omega, status, dl);
   }
@@ -127,17 +128,19 @@ int resident_per_cu(bool masked, int vq)
 
 }  // namespace
 
-// Launch one red-black iteration (red pass, black pass) on the split layout.
-hipError_t qesmc_launch_split_iteration(qesmc_ctx *x, bool masked, int t, int itermax, float omega,
-                                        float tol)
+namespace {
+
+// One colour pass over planes [k_lo, k_hi) of the updated range.
+hipError_t launch_split_pass(qesmc_ctx *x, bool masked, int colour, int k_lo, int k_hi, int t, int itermax,
+                             float omega, float tol, int reverse)
 {
   const Dims &d = x->d;
-  if (d.NZ - 2 < 1 || d.NY - 2 < 1) return hipSuccess;
+  if (k_hi <= k_lo) return hipSuccess;
   const int vq = x->tile_v;
   const int kb = x->kplanes < 1 ? 1 : x->kplanes;
   const long long flat = (long long)(d.NY - 2) * d.HP;
   const int tiles_per_plane = (int)((flat + 64LL * vq - 1) / (64LL * vq));
-  const int nkc = (d.NZ - 2 + kb - 1) / kb;
+  const int nkc = (k_hi - k_lo + kb - 1) / kb;
   const long long ntiles = (long long)tiles_per_plane * nkc;
   long long blocks = (ntiles + 3) / 4;
   int wg = x->wg_per_cu < 1 ? 1 : x->wg_per_cu;
@@ -147,29 +150,70 @@ hipError_t qesmc_launch_split_iteration(qesmc_ctx *x, bool masked, int t, int it
   if (blocks > cap) blocks = cap;
   if (blocks < kXcds) blocks = kXcds;  // every XCD's tile range needs a work-group
   const float omo = 1.0f - omega;
+  float *own = x->d_lc[colour], *oth = x->d_lc[colour ^ 1];
+  const float *const *cc = x->d_cc[colour];
+  const unsigned char *mk = x->d_mask[colour];
+#define QESMC_SPLIT_LAUNCH(M, V)                                                                          \
+  hipLaunchKernelGGL((k_sor_split<M, V>), dim3((unsigned)blocks), dim3(256), 0, x->stream, d, colour, kb, \
+                     k_lo, k_hi, tiles_per_plane, ntiles, own, oth, x->d_Rc[colour], mk, x->d_canon, cc[0], \
+                     cc[1], cc[2], cc[3], cc[4], cc[5], omega, omo, tol, x->d_err, t, itermax, x->d_status,   \
+                     reverse)
+  if (masked) {
+    if (vq == 4) QESMC_SPLIT_LAUNCH(true, 4);
+    else if (vq == 2) QESMC_SPLIT_LAUNCH(true, 2);
+    else QESMC_SPLIT_LAUNCH(true, 1);
+  } else {
+    if (vq == 4) QESMC_SPLIT_LAUNCH(false, 4);
+    else if (vq == 2) QESMC_SPLIT_LAUNCH(false, 2);
+    else QESMC_SPLIT_LAUNCH(false, 1);
+  }
+#undef QESMC_SPLIT_LAUNCH
+  return hipGetLastError();
+}
+
+}  // namespace
+
+// Launch one red-black iteration (red pass, black pass) on the split layout.
+//
+// With x->slab_planes = S > 0 the two passes are interleaved over k-slabs of
+// S planes: red over slab s, then black over the same planes shifted down by
+// one (black plane k needs red planes k-1..k+1 of this iteration, so it can
+// run once red has done plane k+1), then red over slab s+1, and so on; the
+// last black launch finishes the range.  Every red update still reads the
+// previous iteration's black values and every black update this iteration's
+// red values, exactly as with two whole-domain passes (a red launch over
+// planes [a, b) reads black planes a-1..b, which no black launch of this
+// iteration has touched yet; a black launch over [a', b-1) reads red planes
+// up to b-1, all relaxed), so results are unchanged.  What changes is where
+// the bytes come from: the black launch re-reads the lambda planes that the
+// red launch just read and wrote, from the Infinity Cache instead of HBM,
+// when a slab's footprint fits in it.
+hipError_t qesmc_launch_split_iteration(qesmc_ctx *x, bool masked, int t, int itermax, float omega,
+                                        float tol)
+{
+  const Dims &d = x->d;
+  if (d.NZ - 2 < 1 || d.NY - 2 < 1) return hipSuccess;
   // Reverse the black pass only when the pass's arrays (lambda and R of both
   // colours and the masks) exceed MI355X's 256 MB Infinity Cache: measured
   // 2.7 % faster per iteration on 1000 x 1000 x 150, 0.8 % slower on
   // 400 x 400 x 120, whose arrays stay cache-resident either way.
   const int reverse = (double)x->nsplit * 2.0 * (4.0 + 4.0 + 1.0) > 256.0 * (1 << 20) ? 1 : 0;
-  for (int colour = 0; colour < 2; ++colour) {
-    float *own = x->d_lc[colour], *oth = x->d_lc[colour ^ 1];
-    const float *const *cc = x->d_cc[colour];
-    const unsigned char *mk = x->d_mask[colour];
-#define QESMC_SPLIT_LAUNCH(M, V)                                                                          \
-  hipLaunchKernelGGL((k_sor_split<M, V>), dim3((unsigned)blocks), dim3(256), 0, x->stream, d, colour, kb, \
-                     tiles_per_plane, ntiles, own, oth, x->d_Rc[colour], mk, x->d_canon, cc[0], cc[1],    \
-                     cc[2], cc[3], cc[4], cc[5], omega, omo, tol, x->d_err, t, itermax, x->d_status, reverse)
-    if (masked) {
-      if (vq == 4) QESMC_SPLIT_LAUNCH(true, 4);
-      else if (vq == 2) QESMC_SPLIT_LAUNCH(true, 2);
-      else QESMC_SPLIT_LAUNCH(true, 1);
-    } else {
-      if (vq == 4) QESMC_SPLIT_LAUNCH(false, 4);
-      else if (vq == 2) QESMC_SPLIT_LAUNCH(false, 2);
-      else QESMC_SPLIT_LAUNCH(false, 1);
-    }
-#undef QESMC_SPLIT_LAUNCH
+  const int lo = 1, hi = d.NZ - 1;  // updated planes [1, NZ-2]
+  const int S = x->slab_planes;
+  if (S <= 0 || S >= hi - lo) {
+    hipError_t e = launch_split_pass(x, masked, 0, lo, hi, t, itermax, omega, tol, reverse);
+    if (e != hipSuccess) return e;
+    return launch_split_pass(x, masked, 1, lo, hi, t, itermax, omega, tol, reverse);
   }
-  return hipGetLastError();
+  int black_lo = lo;
+  for (int a = lo; a < hi; a += S) {
+    const int b = a + S < hi ? a + S : hi;
+    hipError_t e = launch_split_pass(x, masked, 0, a, b, t, itermax, omega, tol, reverse);
+    if (e != hipSuccess) return e;
+    const int black_hi = b == hi ? hi : b - 1;
+    e = launch_split_pass(x, masked, 1, black_lo, black_hi, t, itermax, omega, tol, reverse);
+    if (e != hipSuccess) return e;
+    black_lo = black_hi;
+  }
+  return hipSuccess;
 }

@@ -52,6 +52,32 @@ public:
   /** Select the kernel path (QESMC_PATH_*); the default picks the fastest. */
   void setPath(int path) { path_ = path; }
 
+  /**
+   * The logLawFlag == 1 branch of qesWindsMain.cpp:157-165 in one call: solve,
+   * then Wall::wallLogBC on the solved field (u0 = u; wallLogBC; u = u0) on
+   * the device, with WGD->z0.  Leaves WGD->u0,v0,w0 equal to WGD->u,v,w, as
+   * the reference's branch does.
+   */
+  void solveWithLogLaw(WINDSGeneralData *WGD, const int &itermax)
+  {
+    qesmc_grid grid{domain.nx(), domain.ny(), domain.nz(), domain.dx(), domain.dy(), domain.dz_array.data()};
+    qesmc_coeffs coeffs{WGD->e.data(), WGD->f.data(), WGD->g.data(), WGD->h.data(),
+                        WGD->m.data(), WGD->n.data(), WGD->icellflag.data()};
+    qesmc_params params = runParams(itermax);
+    int rc = qesmc_set_geometry(ctx_, &grid, &coeffs);
+    if (rc == QESMC_OK) rc = qesmc_set_inflow(ctx_, WGD->u0.data(), WGD->v0.data(), WGD->w0.data());
+    if (rc == QESMC_OK) rc = qesmc_set_lambda(ctx_, lambda.data());
+    if (rc == QESMC_OK) rc = qesmc_run(ctx_, &params, &last_);
+    if (rc == QESMC_OK) rc = qesmc_wall_log_bc(ctx_, WGD->z0, nullptr);
+    if (rc == QESMC_OK) rc = qesmc_get_velocity(ctx_, WGD->u.data(), WGD->v.data(), WGD->w.data());
+    if (rc == QESMC_OK) rc = qesmc_get_lambda(ctx_, lambda.data());
+    if (rc != QESMC_OK) QESout::error(std::string("[Solver_HIP] ") + qesmc_status_string(rc));
+    WGD->u0 = WGD->u;
+    WGD->v0 = WGD->v;
+    WGD->w0 = WGD->w;
+    report();
+  }
+
   void solve(WINDSGeneralData *WGD, const int &itermax) override
   {
     qesmc_grid grid{domain.nx(), domain.ny(), domain.nz(), domain.dx(), domain.dy(), domain.dz_array.data()};
@@ -59,6 +85,15 @@ public:
                         WGD->m.data(), WGD->n.data(), WGD->icellflag.data()};
     qesmc_fields fields{WGD->u0.data(), WGD->v0.data(), WGD->w0.data(),
                         WGD->u.data(),  WGD->v.data(),  WGD->w.data()};
+    qesmc_params params = runParams(itermax);
+    const int rc = qesmc_solve(ctx_, &grid, &coeffs, &fields, lambda.data(), &params, &last_);
+    if (rc != QESMC_OK) QESout::error(std::string("[Solver_HIP] ") + qesmc_status_string(rc));
+    report();
+  }
+
+private:
+  qesmc_params runParams(int itermax) const
+  {
     qesmc_params params;
     qesmc_default_params(&params);
     params.omega = omega;
@@ -67,14 +102,16 @@ public:
     params.alpha1 = alpha1;
     params.alpha2 = alpha2;
     params.path = path_;
-    const int rc = qesmc_solve(ctx_, &grid, &coeffs, &fields, lambda.data(), &params, &last_);
-    if (rc != QESMC_OK) QESout::error(std::string("[Solver_HIP] ") + qesmc_status_string(rc));
+    return params;
+  }
+
+  void report() const
+  {
     std::printf("[Solver]\t Residual after %d itertations: %2.9f\n", last_.iterations,
                 last_.final_max_dlambda);
     std::printf("\t\t elapsed time: %f s\n", last_.ms_total * 1e-3);
   }
 
-private:
   qesmc_ctx *ctx_ = nullptr;
   qesmc_result last_{};
   int path_ = QESMC_PATH_AUTO;

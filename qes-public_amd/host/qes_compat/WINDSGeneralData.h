@@ -27,4 +27,5 @@ public:
   std::vector<int> icellflag;           /**< 0 building, 1 fluid, 2 terrain */
   std::vector<float> u0, v0, w0;        /**< initial wind (faces) */
   std::vector<float> u, v, w;           /**< final wind (faces) */
+  float z0{};                           /**< roughness used by wallLogBC (WINDSGeneralData.h:208) */
 };

@@ -72,7 +72,8 @@ class Params(C.Structure):
     _fields_ = [("omega", C.c_float), ("tol", C.c_float), ("itermax", C.c_int),
                 ("alpha1", C.c_int), ("alpha2", C.c_int), ("path", C.c_int),
                 ("check_every", C.c_int), ("flags", C.c_int), ("tile_k", C.c_int),
-                ("tile_v", C.c_int), ("wg_per_cu", C.c_int)]
+                ("tile_v", C.c_int), ("wg_per_cu", C.c_int), ("slab_planes", C.c_int),
+                ("stop_extend", C.c_float)]
 
 
 class Result(C.Structure):
@@ -350,6 +351,13 @@ class Context:
                "qesmc_velocity_gradient")
         return G, ms.value
 
+    def wall_log_bc(self, z0) -> float:
+        """Wall::wallLogBC on the resident corrected field (qesmc_wall_log_bc);
+        returns the device ms of the six passes."""
+        ms = C.c_double()
+        _check(_city_lib().qesmc_wall_log_bc(self._h, float(z0), C.byref(ms)), "qesmc_wall_log_bc")
+        return ms.value
+
     def solve(self, city: City, lam=None, params: Params | None = None, **kw):
         """One-call drop-in (qesmc_solve): returns (u, v, w, lambda, result)."""
         p = params if params is not None else default_params(**kw)
@@ -401,7 +409,8 @@ def autotune(ctx: "Context", iterations=40, params: Params | None = None, **kw) 
 
 
 EXPORTS = EXPORTS + ("qesmc_set_city", "qesmc_set_city_inflow", "qesmc_get_inputs", "qesmc_lambda_combine",
-                     "qesmc_velocity_linf_diff", "qesmc_velocity_gradient")
+                     "qesmc_velocity_linf_diff", "qesmc_velocity_gradient", "qesmc_set_probes",
+                     "qesmc_get_probes", "qesmc_wall_log_bc")
 
 
 class CityShape:
@@ -433,6 +442,12 @@ def _city_lib():
         L.qesmc_velocity_linf_diff.restype = C.c_int
         L.qesmc_velocity_gradient.argtypes = [C.c_void_p, C.c_int, C.c_float, _fp, C.POINTER(C.c_double)]
         L.qesmc_velocity_gradient.restype = C.c_int
+        L.qesmc_set_probes.argtypes = [C.c_void_p, C.c_longlong, C.POINTER(C.c_longlong)]
+        L.qesmc_set_probes.restype = C.c_int
+        L.qesmc_get_probes.argtypes = [C.c_void_p, _fp, _fp, _fp]
+        L.qesmc_get_probes.restype = C.c_int
+        L.qesmc_wall_log_bc.argtypes = [C.c_void_p, C.c_float, C.POINTER(C.c_double)]
+        L.qesmc_wall_log_bc.restype = C.c_int
         L._city_sigs_set = True
     return L
 
@@ -510,6 +525,23 @@ def velocity_linf_diff(a: "Context", b: "Context") -> float:
     out = C.c_double()
     _check(_city_lib().qesmc_velocity_linf_diff(a._h, b._h, C.byref(out)), "qesmc_velocity_linf_diff")
     return out.value
+
+
+def set_probes(ctx: "Context", faces):
+    """Resident list of face indices (i + j nx + k nx ny) whose u, v, w
+    get_probes gathers on the device (qesmc_set_probes)."""
+    idx = np.ascontiguousarray(faces, np.int64)
+    _check(_city_lib().qesmc_set_probes(ctx._h, idx.size, idx.ctypes.data_as(C.POINTER(C.c_longlong))),
+           "qesmc_set_probes")
+    ctx._n_probe = int(idx.size)
+
+
+def get_probes(ctx: "Context"):
+    """u, v, w of the resident corrected field at the probe faces."""
+    n = getattr(ctx, "_n_probe", 0)
+    u, v, w = (np.empty(n, np.float32) for _ in range(3))
+    _check(_city_lib().qesmc_get_probes(ctx._h, _f(u), _f(v), _f(w)), "qesmc_get_probes")
+    return u, v, w
 
 
 UNET_IN_CHANNELS = 8

@@ -57,6 +57,8 @@ int oracle_sor_rb_f64(int nx, int ny, int nz, const float *e, const float *f, co
 double oracle_div_linf(int nx, int ny, int nz, double dx, double dy, const float *dz_array, const float *e,
                        const float *f, const float *g, const float *h, const float *m, const float *n,
                        const int *icellflag, const float *u, const float *v, const float *w, int physical);
+long oracle_wall_log_bc(int nx, int ny, int nz, float dx, float dz, float z0, const int *icellflag, float *u,
+                        float *v, float *w);
 void oracle_velocity_gradient(int nx, int ny, int nz, float dx, float dy, float dz, const float *dz_array,
                               int uniform_z, const int *icellflag, const float *u, const float *v, const float *w,
                               float *G);
@@ -182,6 +184,12 @@ int main(int argc, char **argv)
                                        w.data(), 1)));
   oracle_velocity_gradient(nx, ny, nz, dx, dx, 1.0f, dz.data(), 0, flag.data(), u.data(), v.data(), w.data(),
                            G.data());
+  {
+    // wallLogBC restatement (reads neighbour faces at +-1, +-nx, +-nx*ny of
+    // every wall face, including the wrapped ones at the domain edges)
+    std::vector<float> lu = u, lv = v, lw = w;
+    EXPECT(oracle_wall_log_bc(nx, ny, nz, dx, 1.0f, 0.1f, flag.data(), lu.data(), lv.data(), lw.data()) > 0);
+  }
 
   // ---- 2. a valid workspace file round trip --------------------------------
   const std::string good = dir + "/good.nc";

@@ -21,7 +21,25 @@ extern "C" int oracle_solve_rb_f32(int nx, int ny, int nz, float dx, float dy, c
                                    float *lambda_old, float omega, float tol, int itermax, int alpha1,
                                    int alpha2, float *u, float *v, float *w, float *max_error_out);
 
+extern "C" long oracle_wall_log_bc(int nx, int ny, int nz, float dx, float dz, float z0, const int *icellflag,
+                                   float *u, float *v, float *w);
+
 static int failures = 0;
+
+// Device and host math libraries may differ by a few ulp in atan2f / cosf /
+// sinf, which wallLogBC evaluates per wall face.
+static void expect_close(const std::vector<float> &a, const std::vector<float> &b, const char *what)
+{
+  size_t bad = 0;
+  for (size_t q = 0; q < a.size(); ++q) {
+    const double d = std::fabs((double)a[q] - (double)b[q]);
+    if (!(d <= 2e-6 * (1.0 + std::fabs((double)b[q])))) ++bad;
+  }
+  if (bad) {
+    std::printf("MISMATCH %s: %zu of %zu entries beyond 2e-6 relative\n", what, bad, a.size());
+    ++failures;
+  }
+}
 
 static void expect_equal(const std::vector<float> &a, const std::vector<float> &b, const char *what)
 {
@@ -93,6 +111,29 @@ int main()
     std::printf("step %d: %d iterations, %.3f ms on device, %.1f GLUPS\n", step, hip->lastResult().iterations,
                 hip->lastResult().ms_total, hip->lastResult().glups);
   }
+  // the logLawFlag == 1 branch (qesWindsMain.cpp:157-165): solve, then
+  // wallLogBC on the solved field, u0 left equal to u
+  set_inflow(wgd, dom, 285.0f, bl);
+  wgd.z0 = 0.1f;
+  const std::vector<float> u0 = wgd.u0, v0 = wgd.v0, w0 = wgd.w0;
+  solver->resetLambda();
+  std::fill(lam.begin(), lam.end(), 0.f);
+  hip->solveWithLogLaw(&wgd, 200);
+  oracle_solve_rb_f32(dom.nx(), dom.ny(), dom.nz(), dom.dx(), dom.dy(), dom.dz_array.data(), wgd.e.data(),
+                      wgd.f.data(), wgd.g.data(), wgd.h.data(), wgd.m.data(), wgd.n.data(), wgd.icellflag.data(),
+                      u0.data(), v0.data(), w0.data(), R.data(), lam.data(), lam_old.data(), 1.78f, 1e-9f, 200, 1,
+                      1, u.data(), v.data(), w.data(), &me);
+  float dzmin = dom.dz_array[0];
+  for (float d : dom.dz_array) dzmin = d < dzmin ? d : dzmin;
+  const long nwall = oracle_wall_log_bc(dom.nx(), dom.ny(), dom.nz(), dom.dx(), dzmin, wgd.z0,
+                                        wgd.icellflag.data(), u.data(), v.data(), w.data());
+  expect_close(wgd.u, u, "u (log law)");
+  expect_close(wgd.v, v, "v (log law)");
+  expect_close(wgd.w, w, "w (log law)");
+  expect_equal(wgd.u0, wgd.u, "u0 == u after the log-law branch");
+  expect_equal(wgd.w0, wgd.w, "w0 == w after the log-law branch");
+  expect_equal(solver->getLambda(), lam, "lambda (log law)");
+  std::printf("log-law step: %ld wall faces, %d iterations\n", nwall, hip->lastResult().iterations);
   std::printf(failures ? "FAIL\n" : "PASS\n");
   return failures ? 1 : 0;
 }

@@ -155,3 +155,41 @@ def test_batch_larger_than_the_infinity_cache(qesmc, oracle):
     finally:
         for c in ctxs:
             c.close()
+
+
+@pytest.mark.timeout(600)
+def test_batch_of_the_benchmark_cities_equals_oracle(qesmc, oracle):
+    """The benchmark's own 64-city batch workload (bench.batch_cities: 64-256 x
+    64-256 x 64-96 cells, up to 120 buildings, directions 5.625 deg apart), the
+    first 16 members at full size: inputs built by the oracle's restatement of
+    the reference's builders, 10 batched iterations replayed as HIP graphs of 5,
+    each member bitwise equal to the oracle's fp32 red-black solve."""
+    import os
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    if root not in sys.path:
+        sys.path.insert(0, root)
+    import bench
+
+    specs = bench.batch_cities(64)[:16]
+    probs = [oracle.build_city(ncx, ncy, 2.0, dz, [tuple(b) for b in blds], inflow)
+             for ncx, ncy, dz, blds, inflow in specs]
+    assert max(p.ncell for p in probs) > 2_000_000
+    ctxs = [qesmc.Context(0) for _ in probs]
+    try:
+        for c, p in zip(ctxs, probs):
+            c.set_geometry(p)
+            c.set_inflow(p.u0, p.v0, p.w0)
+            c.set_lambda(None)
+        res, _ = qesmc.run_batch(ctxs, itermax=10, tol=1e-9, check_every=5, use_graph=True)
+        for q, (c, p, r) in enumerate(zip(ctxs, probs, res)):
+            u, v, w, lam, its, me = oracle.solve_rb_f32(p, itermax=10)
+            assert r["iterations"] == its == 10, q
+            assert r["final_max_dlambda"] == np.float32(me), q
+            gu, gv, gw = c.get_velocity()
+            for a, b in ((gu, u), (gv, v), (gw, w), (c.get_lambda(), lam)):
+                assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), q
+    finally:
+        for c in ctxs:
+            c.close()

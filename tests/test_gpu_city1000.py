@@ -1,0 +1,66 @@
+"""The 1000 x 1000 x 150 city (151 M cells, 3000 lognormal buildings) against
+its FP64 fixture (tests/golden/city1000_fp64.npz: sampled u, v, w of the FP64
+red-black fixed point for winds from 270 and 180 deg, written by
+tests/golden/make_city1000_fp64.py with the oracle).  The accuracy bar of the
+large-domain ensemble: the fp32 device field within 1e-4 m/s (L_inf over the
+sampled u, v, w) of the FP64 solution, from a cold start at the iteration
+count the benchmark reports, and for another direction from the superposition
+of the two basis solutions (the discrete problem is linear in the inflow)."""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FIXTURE = os.path.join(ROOT, "tests", "golden", "city1000_fp64.npz")
+COLD_ITERS = {270.0: 64000, 180.0: 64000}   # cold T_tol of the benchmark's run, rounded up
+SUPERPOSED = {45.0: 200}                    # superposed direction: iterations from the combination
+
+
+def _bench():
+    if ROOT not in sys.path:
+        sys.path.insert(0, ROOT)
+    import bench
+    return bench
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.skipif(not os.path.exists(FIXTURE), reason="FP64 fixture not generated (tests/golden/make_city1000_fp64.py)")
+def test_city1000_within_1e4_of_fp64(qesmc):
+    b = _bench()
+    Q = qesmc
+    F = np.load(FIXTURE)
+    ncx, ncy, dz, blds, u_ref, _ = b.large_city()
+    assert np.array_equal(F["dz_array"], dz)
+    assert np.array_equal(F["buildings"], np.array(blds, np.float32))
+    faces, ref = b.c5_fixture_probes(F, ncx + 1, ncy + 1)
+    ctx = [Q.Context(0) for _ in range(3)]
+    work, bx, by = ctx
+    try:
+        for c in ctx:
+            c.set_city(ncx, ncy, 1.0, dz, blds, (1, u_ref, 10.0, 0.5, 270.0))
+        Q.set_probes(work, faces)
+        for (wd, its), basis in zip(COLD_ITERS.items(), (bx, by)):
+            work.set_city_inflow((1, u_ref, 10.0, 0.5, wd))
+            work.set_lambda(None)
+            r = work.run(itermax=its, tol=-1.0)
+            assert r["iterations"] == its
+            err = b.probe_error(work, b.c5_reference(ref, wd))
+            print(f"{wd} deg cold, {its} iterations: sampled error {err:.3e} m/s")
+            assert err <= 1e-4, (wd, err)
+            Q.lambda_combine(basis, [work], np.array([1.0], np.float32))
+        for wd, its in SUPERPOSED.items():
+            th = math.radians(270.0 - wd)
+            work.set_city_inflow((1, u_ref, 10.0, 0.5, wd))
+            Q.lambda_combine(work, [bx, by], np.array([math.cos(th), math.sin(th)], np.float32))
+            work.run(itermax=its, tol=-1.0)
+            err = b.probe_error(work, b.c5_reference(ref, wd))
+            print(f"{wd} deg superposed, {its} iterations: sampled error {err:.3e} m/s")
+            assert err <= 1e-4, (wd, err)
+    finally:
+        for c in ctx:
+            c.close()

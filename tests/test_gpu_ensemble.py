@@ -79,3 +79,28 @@ def test_direction_superposition_warm_start(qesmc, oracle):
     assert err(warm30) <= 1e-4, err(warm30)
     assert err(cold30) > 10 * err(warm30), (err(cold30), err(warm30))
     assert warm_it < 0.5 * cold_it < 5000, (warm_it, cold_it)
+
+
+def test_probes_gather_the_resident_field(qesmc):
+    """qesmc_get_probes returns u, v, w of the corrected field at the listed
+    faces, in list order (repeats and the first and last face included), and
+    refuses indices outside the face arrays."""
+    dz, blds = city_inputs(qesmc)
+    with qesmc.Context(0) as a:
+        a.set_city(NCX, NCY, DX, dz, blds, (1, U_REF, 10.0, 0.5, 123.0))
+        a.run(itermax=40)
+        u, v, w = a.get_velocity()
+        n = u.size
+        idx = np.random.default_rng(4).integers(0, n, 5000)
+        idx = np.concatenate([[0, n - 1, 7, 7], idx]).astype(np.int64)
+        qesmc.set_probes(a, idx)
+        pu, pv, pw = qesmc.get_probes(a)
+        assert np.array_equal(pu, u[idx]) and np.array_equal(pv, v[idx]) and np.array_equal(pw, w[idx])
+        # a later run: the same list reads the new field
+        a.run(itermax=10)
+        u2, _, w2 = a.get_velocity()
+        pu2, _, pw2 = qesmc.get_probes(a)
+        assert np.array_equal(pu2, u2[idx]) and np.array_equal(pw2, w2[idx])
+        for bad in ([n], [-1]):
+            with pytest.raises(qesmc.QesmcError):
+                qesmc.set_probes(a, np.array(bad, np.int64))

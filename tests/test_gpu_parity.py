@@ -239,11 +239,11 @@ def test_city_1000x1000x150_bit_exact(oracle, qesmc):
     assert p.ncell == 1000 * 1000 * 151
     ou, ov, ow, olam, oit, ome = oracle.solve_rb_f32(p, itermax=12)
     with qesmc.Context(0) as ctx:
-        for path in (SPLIT, FUSED, PLAIN):
-            u, v, w, lam, res = ctx.solve(p, path=path, itermax=12)
+        for path, slab in ((SPLIT, 0), (SPLIT, -1), (SPLIT, 20), (FUSED, 0), (PLAIN, 0)):
+            u, v, w, lam, res = ctx.solve(p, path=path, itermax=12, slab_planes=slab)
             assert res["iterations"] == oit == 12 and res["path"] == path
             assert np.float32(res["final_max_dlambda"]) == np.float32(ome)
-            assert_same(lam, olam, f"lambda path {path}")
+            assert_same(lam, olam, f"lambda path {path} slab {slab}")
             assert_same(u, ou, "u")
             assert_same(v, ov, "v")
             assert_same(w, ow, "w")
@@ -258,3 +258,14 @@ def test_tile_parameters_never_change_results(gpu_ctx, oracle, qesmc, tile):
     assert res["compressed"] == 1
     check_solve(gpu_ctx, oracle, p, PLAIN, itermax=12, tile_k=tk)
     assert check_solve(gpu_ctx, oracle, p, FUSED, itermax=40, tile_k=tk)["tile_k"] == tk
+
+
+@pytest.mark.parametrize("slab", [1, 2, 3, 7, 18, 19, 40])
+@pytest.mark.parametrize("check_every", [1, 64])
+def test_slab_schedule_never_changes_results(gpu_ctx, oracle, qesmc, slab, check_every):
+    """The interleaved red/black schedule over k-slabs (slab_planes) reorders
+    the launches only: bit-exact against the oracle for slabs of one plane up
+    to more than the domain, with a tol that stops the solve on the way."""
+    p = random_city(oracle, qesmc, 91, 90, 70, 21)
+    check_solve(gpu_ctx, oracle, p, SPLIT, itermax=300, tol=2e-3, slab_planes=slab, check_every=check_every)
+    check_solve(gpu_ctx, oracle, p, SPLIT, itermax=25, slab_planes=slab, tile_k=3, tile_v=1)

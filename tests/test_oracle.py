@@ -620,3 +620,123 @@ def test_rotated_building_matches_half_plane_test(oracle, rot):
         got = fl[k] == 0
         assert np.array_equal(got[~near], want[~near]), (k, np.argwhere(got != want)[:5])
     assert inside.sum() > 100  # about L W / dx^2 = 112 cells per layer at dx 1.5: not vacuous
+
+
+# ---------------------------------------------------------------------------
+# Wall::wallLogBC on stair-step walls (oracle_wall_log_bc)
+# ---------------------------------------------------------------------------
+def _log_law_closed_form(V, d1, d2, z0, kappa=0.4, steps=20):
+    """Final wall-parallel speed of the 20-step fixed point of Wall.cpp:677-797
+    in closed form: u*_{n+1} = a - b u*_n with a = kappa V / ln(d1/z0),
+    b = ln(d2/d1) / ln(d1/z0), u*_0 = 0.1, so
+    u*_n = a (1 - (-b)^n) / (1 + b) + (-b)^n 0.1, and the last step writes
+    V - u*_{steps-1} ln(d2/d1) / kappa."""
+    L1, L2 = np.log(d1 / z0), np.log(d2 / d1)
+    a, b = kappa * V / L1, L2 / L1
+    n = steps - 1
+    us = a * (1 - (-b) ** n) / (1 + b) + (-b) ** n * 0.1
+    return V - us * L2 / kappa
+
+
+def _box_city(O, dz_kind, z0):
+    """One box building on flat ground, random wall-parallel velocities on
+    every face (so each pass's result depends on which neighbour it reads)."""
+    ncx, ncy, ncz, dx = 24, 20, 12, 2.0
+    dz = O.dz_uniform(ncz, 1.0) if dz_kind == "uniform" else O.dz_stretched(1.0 * 1.1 ** np.arange(ncz))
+    p = O.build_city(ncx, ncy, dx, dz, [(16.0, 12.0, 14.0, 10.0, 6.5, 0.0, 0.0)], (1, 4.0, 10.0, z0, 250.0))
+    rng = np.random.default_rng(3)
+    u, v, w = (rng.uniform(0.5, 4.0, p.nface).astype(np.float32) * s for s in (1.0, -1.0, 0.5))
+    NX, NY, NZ = p.nx - 1, p.ny - 1, p.nz - 1
+    solid = (p.icellflag.reshape(NZ, NY, NX)[1:] == 0)
+    ks, js, is_ = np.nonzero(solid)
+    box = (is_.min(), is_.max() + 1, js.min(), js.max() + 1, ks.max() + 1)  # i0, i1, j0, j1, top layer
+    assert solid.sum() == (box[1] - box[0]) * (box[3] - box[2]) * box[4]  # a box
+    return p, u, v, w, box
+
+
+@pytest.mark.parametrize("dz_kind,z0", [("uniform", 0.1), ("stretched", 0.1), ("uniform", 0.3)])
+def test_wall_log_bc_closed_form(oracle, dz_kind, z0):
+    """Each of the six passes against the closed form of its fixed point, on a
+    face that only that pass touches, with the neighbour the reference reads
+    (Wall.cpp:681-797: +nx*ny, -nx*ny, +1, -1, +nx, -nx) and the components it
+    writes (normal 0, parallel = speed x cos/sin of the neighbour's
+    direction); the face counts of a box building (every k = 1 fluid cell has
+    the ground below it); a face in two lists takes the later pass's result
+    computed from the neighbour the earlier pass rewrote (pass order); faces in
+    no list are untouched.  z0 = 0.3 m makes the recurrence diverge (b > 1),
+    which pins the step count."""
+    O = oracle
+    p, u, v, w, (i0, i1, j0, j1, H) = _box_city(O, dz_kind, z0)
+    nx, ny = p.nx, p.ny
+    NX, NY = nx - 1, ny - 1
+    fz = nx * ny
+    F = lambda i, j, k: i + j * nx + k * fz  # noqa: E731
+    dz = float(np.min(p.dz_array))
+    dx = p.dx
+    U, Vv, Wv, nfaces = O.wall_log_bc(p, u, v, w, z0)
+    L, Wd = i1 - i0, j1 - j0
+    assert nfaces == NX * NY + 2 * H * (L + Wd)
+
+    def check(f, nb, comps, normal, d, src=(u, v, w), tol=2e-5):
+        A, B = (src[c][nb].astype(np.float64) for c in comps)
+        V = np.hypot(A, B)
+        m = _log_law_closed_form(V, 0.5 * d, 1.5 * d, z0)
+        th = np.arctan2(B, A)
+        out = (U, Vv, Wv)
+        assert out[normal][f] == 0.0
+        for c, want in zip(comps, (m * np.cos(th), m * np.sin(th))):
+            assert abs(out[c][f] - want) <= tol * max(1.0, abs(want)), (f, c, out[c][f], want)
+
+    jm, km = (j0 + j1) // 2, max(2, H // 2)
+    im = (i0 + i1) // 2
+    check(F(3, 4, 1), F(3, 4, 2), (0, 1), 2, dz)                 # below (ground)
+    check(F(im, jm, H + 1), F(im, jm, H + 2), (0, 1), 2, dz)     # below (roof)
+    check(F(i1, jm, km), F(i1 + 1, jm, km), (1, 2), 0, dx)       # back: solid at i-1
+    check(F(i0 - 1, jm, km), F(i0 - 2, jm, km), (1, 2), 0, dx)   # front: solid at i+1
+    check(F(im, j1, km), F(im, j1 + 1, km), (0, 2), 1, dx)       # right: solid at j-1
+    check(F(im, j0 - 1, km), F(im, j0 - 2, km), (0, 2), 1, dx)   # left: solid at j+1
+    # a k = 1 cell beside the building is in the below and the back list: the
+    # back pass reads its neighbour after the below pass rewrote it
+    nb = F(i1 + 1, jm, 1)
+    mid = [a.copy() for a in (u, v, w)]
+    A, B = (mid[c][F(i1 + 1, jm, 2)].astype(np.float64) for c in (0, 1))
+    m = _log_law_closed_form(np.hypot(A, B), 0.5 * dz, 1.5 * dz, z0)
+    th = np.arctan2(B, A)
+    mid[0][nb], mid[1][nb], mid[2][nb] = m * np.cos(th), m * np.sin(th), 0.0
+    check(F(i1, jm, 1), nb, (1, 2), 0, dx, src=mid, tol=1e-4)
+    # faces of cells in no list keep their values
+    far = F(3, 4, 5)
+    assert U[far] == u[far] and Vv[far] == v[far] and Wv[far] == w[far]
+
+
+def test_wall_log_bc_refuses_cut_cells(oracle):
+    O = oracle
+    p, u, v, w, _ = _box_city(O, "uniform", 0.1)
+    flag = p.icellflag.copy()
+    flag[flag.size // 2] = 7
+    q = O.Problem(p.nx, p.ny, p.nz, p.dx, p.dy, p.dz_array, *p.coeffs, flag, p.u0, p.v0, p.w0)
+    with pytest.raises(ValueError):
+        O.wall_log_bc(q, u, v, w, 0.1)
+
+
+@pytest.mark.parametrize("wind", [45.0, 135.0, 333.0])
+def test_fp64_solution_is_linear_in_the_wind_direction(oracle, wind):
+    """The premise of the large-domain FP64 fixture: the discrete problem is
+    linear in (u0, v0, w0) and a single-sensor inflow from direction d is
+    cos(t) x the inflow from 270 deg + sin(t) x the inflow from 180 deg,
+    t = 270 - d (WindProfilerSensorType.cpp:571-621), so the converged FP64
+    field of any direction is that combination of the two basis fields, up to
+    the fp32 rounding of the inflow values (~1e-7 relative)."""
+    O = oracle
+    fields = {}
+    for d in (270.0, 180.0, wind):
+        p = small_city(O, 30, 24, 14, wind=d)
+        u, v, w, lam, its, me = O.solve_rb_f64(p, tol=1e-11)
+        fields[d] = np.stack([u, v, w])
+    th = np.radians(270.0 - wind)
+    comb = np.cos(th) * fields[270.0] + np.sin(th) * fields[180.0]
+    scale = np.abs(fields[wind]).max()
+    assert np.abs(comb - fields[wind]).max() <= 2e-6 * scale
+    # and a wrong combination (sign of the second basis flipped) is far off
+    bad = np.cos(th) * fields[270.0] - np.sin(th) * fields[180.0]
+    assert np.abs(bad - fields[wind]).max() > 1e-2 * scale
