@@ -40,6 +40,8 @@ extern "C" {
 #define QESMC_ERR_UNSUPPORTED (-8)  /* option not available for these inputs   */
 #define QESMC_ERR_IO (-9)           /* file missing, unwritable or not a NetCDF
                                        classic (CDF-1/CDF-2) file               */
+#define QESMC_ERR_LIMIT (-10)       /* beyond the fixed capacity of an entry
+                                       point that allocates nothing             */
 
 /* ---- grid / input / output descriptors ---------------------------------- */
 
@@ -165,6 +167,19 @@ int qesmc_set_lambda(qesmc_ctx *ctx, const float *lambda);
 int qesmc_run(qesmc_ctx *ctx, const qesmc_params *params, qesmc_result *result);
 int qesmc_get_velocity(qesmc_ctx *ctx, float *u, float *v, float *w);
 int qesmc_get_lambda(qesmc_ctx *ctx, float *lambda);
+
+/* qesmc_run without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  qesmc_run allocates
+ * its per-iteration convergence rows when a run needs more of them than any
+ * earlier run of the context; this variant takes them from a fixed
+ * per-device array of the library (16.8 MB, held until the process ends) and
+ * calls no allocating routine, so params->itermax is limited to 65536
+ * (QESMC_ERR_LIMIT beyond).  The fused path needs its second lambda buffers
+ * from an earlier qesmc_run with QESMC_PATH_FUSED (QESMC_ERR_STATE
+ * otherwise).  Results are qesmc_run's, bit for bit.  All calls on one
+ * device share the array: calls that overlap on one device race. */
+int qesmc_run_static(qesmc_ctx *ctx, const qesmc_params *params, qesmc_result *result);
 /* Max |div| over interior fluid cells of the corrected field, FP64 on the
  * device: physical = 0 uses the reference operator R(u)/(-2 alpha1^2),
  * physical = 1 the dimensionally consistent one (vertical term / dz_k). */

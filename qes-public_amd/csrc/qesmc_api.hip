@@ -206,6 +206,7 @@ const char *qesmc_status_string(int s)
   case QESMC_ERR_STATE: return "inputs not set";
   case QESMC_ERR_UNSUPPORTED: return "unsupported option";
   case QESMC_ERR_IO: return "file missing, unwritable or not a NetCDF classic file";
+  case QESMC_ERR_LIMIT: return "beyond the fixed capacity of an entry point that allocates nothing";
   default: return "unknown status";
   }
 }
@@ -252,6 +253,9 @@ int qesmc_create(qesmc_ctx **out, int device)
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
       x->n_cu = ncu;
   }
+  // the no-allocation entry point's per-device scratch address (errors are
+  // reported by qesmc_run_static itself, which repeats the lookup)
+  (void)qesmc::static_scratch_init(device);
   if (fail(hipHostMalloc((void **)&x->h_poll, (2 * qesmc::kErrSlots + 8) * sizeof(unsigned), hipHostMallocDefault))) goto bad;
   if (fail(dalloc(&x->d_status, qesmc::kStatusWords))) goto bad;
   if (fail(dalloc(&x->d_norm, 1))) goto bad;

@@ -154,6 +154,7 @@ struct RunState {
 int run_prepare(qesmc_ctx *x, const qesmc_params *pp, RunState &st);
 int run_finish(qesmc_ctx *x, const RunState &st, int launched, qesmc_result *res);
 int run_solve(qesmc_ctx *x, const qesmc_params *pp, qesmc_result *res);
+int static_scratch_init(int device);
 float row_max(const unsigned *row);
 
 // Geometry set-up shared by qesmc_set_geometry and the device city builder:

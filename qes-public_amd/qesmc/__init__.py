@@ -28,6 +28,7 @@ ERR_NONFINITE = -6
 ERR_STATE = -7
 ERR_UNSUPPORTED = -8
 ERR_IO = -9
+ERR_LIMIT = -10
 
 PATH_AUTO = 0
 PATH_PLAIN = 1
@@ -40,7 +41,7 @@ EXPORTS = (
     "qesmc_create", "qesmc_destroy", "qesmc_default_params", "qesmc_status_string",
     "qesmc_solve", "qesmc_set_geometry", "qesmc_set_inflow", "qesmc_set_lambda", "qesmc_run",
     "qesmc_get_velocity", "qesmc_get_lambda", "qesmc_divergence_linf", "qesmc_city_build",
-    "qesmc_city_random_buildings",
+    "qesmc_city_random_buildings", "qesmc_run_static",
 )
 
 
@@ -121,6 +122,7 @@ def lib():
     L.qesmc_set_inflow.argtypes = [P, _fp, _fp, _fp]
     L.qesmc_set_lambda.argtypes = [P, _fp]
     L.qesmc_run.argtypes = [P, C.POINTER(Params), C.POINTER(Result)]
+    L.qesmc_run_static.argtypes = [P, C.POINTER(Params), C.POINTER(Result)]
     L.qesmc_get_velocity.argtypes = [P, _fp, _fp, _fp]
     L.qesmc_get_lambda.argtypes = [P, _fp]
     L.qesmc_divergence_linf.argtypes = [P, C.c_int, C.POINTER(C.c_double)]
@@ -295,6 +297,15 @@ class Context:
         rc = lib().qesmc_run(self._h, C.byref(p), C.byref(r))
         if rc != OK and not (allow_nonfinite and rc == ERR_NONFINITE):
             raise QesmcError(rc, "qesmc_run")
+        return r.as_dict()
+
+    def run_static(self, params: Params | None = None, allow_nonfinite=False, **kw) -> dict:
+        """qesmc_run without device allocation (qesmc_run_static): itermax <= 65536."""
+        p = params if params is not None else default_params(**kw)
+        r = Result()
+        rc = lib().qesmc_run_static(self._h, C.byref(p), C.byref(r))
+        if rc != OK and not (allow_nonfinite and rc == ERR_NONFINITE):
+            raise QesmcError(rc, "qesmc_run_static")
         return r.as_dict()
 
     def get_velocity(self):
