@@ -40,9 +40,14 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "qes-public_amd"))
 # CPU baseline: one OpenMP thread per physical core, packed; the OpenMP
-# runtime reads these when the oracle library is first loaded
+# runtime reads these when the oracle library is first loaded (and then binds
+# this thread to one core, so the CPUs the process may use are noted first)
 os.environ.setdefault("OMP_PROC_BIND", "close")
 os.environ.setdefault("OMP_PLACES", "cores")
+try:
+    START_CPUS = sorted(os.sched_getaffinity(0))
+except AttributeError:
+    START_CPUS = list(range(os.cpu_count() or 1))
 
 import numpy as np  # noqa: E402
 
@@ -114,14 +119,17 @@ def _cpu_model():
     return None
 
 
-def physical_cores():
+def physical_cores(cpus=None):
     """Physical cores among the CPUs this process may run on (sibling
-    hyperthreads counted once), from sysfs topology."""
+    hyperthreads counted once), from sysfs topology.  `cpus` defaults to the
+    process's affinity; bench.py records it at start-up, before the OpenMP
+    runtime binds the main thread to one place."""
     seen = set()
-    try:
-        cpus = sorted(os.sched_getaffinity(0))
-    except AttributeError:
-        cpus = list(range(os.cpu_count() or 1))
+    if cpus is None:
+        try:
+            cpus = sorted(os.sched_getaffinity(0))
+        except AttributeError:
+            cpus = list(range(os.cpu_count() or 1))
     for c in cpus:
         base = "/sys/devices/system/cpu/cpu%d/topology/" % c
         try:
@@ -163,7 +171,7 @@ def cpu_baseline_resident(ctx, desc, samples=5, per_sample=4, serial_iters=1):
         serial = p.interior_lups() * it_s / (time.perf_counter() - t0) / 1e9
     finally:
         O.set_num_threads(threads)
-    phys, logical = physical_cores()
+    phys, logical = physical_cores(START_CPUS)
     glups = float(np.median(rates))
     its = 500
     t_ref_ms = (r_s + p.interior_lups() * its / (glups * 1e9)) * 1e3

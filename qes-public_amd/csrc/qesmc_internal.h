@@ -60,11 +60,14 @@ __device__ inline bool gate_pass(float g, int t, float tol, unsigned *status)
   const float e = __uint_as_float(status[kStopExtend]);
   if (!(e > 0.f)) return g > tol;
   const unsigned tu = (unsigned)t;
-  if (tu >= *(volatile unsigned *)(status + kStopAt)) return false;
+  const unsigned recorded = *(volatile unsigned *)(status + kStopAt);
+  if (tu >= recorded) return false;
   if (g > tol) return true;
   const unsigned end = tu + (unsigned)ceilf(e * (float)t);
   if (end <= tu) return false;
-  if ((threadIdx.x & 63) == 0) atomicMin(status + kStopAt, end);
+  // record only while unset: every later iteration proposes a later end, and
+  // thousands of waves contending for one address serialise at the memory side
+  if (recorded > end && (threadIdx.x & 63) == 0) atomicMin(status + kStopAt, end);
   return true;
 }
 
