@@ -287,6 +287,21 @@ int qesmc_velocity_gradient(qesmc_ctx *ctx, int uniform_z, float dz, float *G, d
  * passes. */
 int qesmc_wall_log_bc(qesmc_ctx *ctx, float z0, double *ms);
 
+/* Over-relaxation factor for a field accuracy (no counterpart in the
+ * reference, which fixes omega = 1.78f, Solver.h:83).  Estimates rho_J, the
+ * spectral radius of the Jacobi iteration of the resident problem (fluid
+ * cells the sweep relaxes, the k = 0 mirror included), by `steps` Lanczos
+ * steps in FP64 on its symmetrised form from a smooth start, and returns
+ * omega = 2 - 1.2 (2 - omega_opt) with Young's omega_opt = 2 / (1 +
+ * sqrt(1 - rho_J^2)) for red-black SOR: slightly below the optimum, since
+ * fp32 rounding noise is amplified as omega nears 2 and past the optimum the
+ * slowest modes only oscillate.  Few steps underestimate rho_J (the Ritz
+ * value approaches it from below), which errs toward a smaller, safer
+ * omega.  rho or omega may be
+ * NULL.  The converged solution does not depend on omega; results at the
+ * reference's omega (parity) are unaffected. */
+int qesmc_estimate_omega(qesmc_ctx *ctx, int steps, double *rho, float *omega);
+
 /* ---- ensembles on one geometry (no counterpart in the reference) ----------
  *
  * The discrete problem is linear in (u0, v0, w0), and a single-sensor inflow

@@ -362,6 +362,13 @@ class Context:
                "qesmc_velocity_gradient")
         return G, ms.value
 
+    def estimate_omega(self, steps=200):
+        """(rho_J, omega) of the resident problem (qesmc_estimate_omega)."""
+        rho, om = C.c_double(), C.c_float()
+        _check(_city_lib().qesmc_estimate_omega(self._h, int(steps), C.byref(rho), C.byref(om)),
+               "qesmc_estimate_omega")
+        return rho.value, om.value
+
     def wall_log_bc(self, z0) -> float:
         """Wall::wallLogBC on the resident corrected field (qesmc_wall_log_bc);
         returns the device ms of the six passes."""
@@ -421,7 +428,7 @@ def autotune(ctx: "Context", iterations=40, params: Params | None = None, **kw) 
 
 EXPORTS = EXPORTS + ("qesmc_set_city", "qesmc_set_city_inflow", "qesmc_get_inputs", "qesmc_lambda_combine",
                      "qesmc_velocity_linf_diff", "qesmc_velocity_gradient", "qesmc_set_probes",
-                     "qesmc_get_probes", "qesmc_wall_log_bc")
+                     "qesmc_get_probes", "qesmc_wall_log_bc", "qesmc_estimate_omega")
 
 
 class CityShape:
@@ -457,6 +464,8 @@ def _city_lib():
         L.qesmc_set_probes.restype = C.c_int
         L.qesmc_get_probes.argtypes = [C.c_void_p, _fp, _fp, _fp]
         L.qesmc_get_probes.restype = C.c_int
+        L.qesmc_estimate_omega.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_float)]
+        L.qesmc_estimate_omega.restype = C.c_int
         L.qesmc_wall_log_bc.argtypes = [C.c_void_p, C.c_float, C.POINTER(C.c_double)]
         L.qesmc_wall_log_bc.restype = C.c_int
         L._city_sigs_set = True

@@ -4,8 +4,12 @@ red-black fixed point for winds from 270 and 180 deg, written by
 tests/golden/make_city1000_fp64.py with the oracle).  The accuracy bar of the
 large-domain ensemble: the fp32 device field within 1e-4 m/s (L_inf over the
 sampled u, v, w) of the FP64 solution, from a cold start at the iteration
-count the benchmark reports, and for another direction from the superposition
-of the two basis solutions (the discrete problem is linear in the inflow)."""
+count the benchmark reports (at the reference's omega 1.78 and at 1.99, just
+below the 1.9912 qesmc_estimate_omega gives for this grid: closer to 2 the
+fp32 rounding noise of lambda, which SOR amplifies as omega nears 2, holds
+the sampled error at 1.0-1.3e-4), and for other directions from the
+superposition of the two basis solutions (the discrete problem is linear in
+the inflow)."""
 import math
 import os
 import sys
@@ -17,8 +21,12 @@ pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 FIXTURE = os.path.join(ROOT, "tests", "golden", "city1000_fp64.npz")
-COLD_ITERS = {270.0: 64000, 180.0: 64000}   # cold T_tol of the benchmark's run, rounded up
-SUPERPOSED = {45.0: 200}                    # superposed direction: iterations from the combination
+# iterations the benchmark reports: T_tol of the reference's omega from a cold
+# start, and of omega 1.99 for the two basis directions, which the ensemble
+# then runs 50 % further so that their combinations stay within 1e-4 m/s
+COLD_REFERENCE = (270.0, 1.78, 56300)
+BASIS = {270.0: (1.99, 2500 * 3 // 2), 180.0: (1.99, 2350 * 3 // 2)}
+SUPERPOSED = {45.0: 0, 135.0: 0}      # iterations from the combination
 
 
 def _bench():
@@ -44,20 +52,26 @@ def test_city1000_within_1e4_of_fp64(qesmc):
         for c in ctx:
             c.set_city(ncx, ncy, 1.0, dz, blds, (1, u_ref, 10.0, 0.5, 270.0))
         Q.set_probes(work, faces)
-        for (wd, its), basis in zip(COLD_ITERS.items(), (bx, by)):
+        wd, om, its = COLD_REFERENCE
+        work.set_city_inflow((1, u_ref, 10.0, 0.5, wd))
+        work.set_lambda(None)
+        assert work.run(itermax=its, tol=-1.0, omega=om)["iterations"] == its
+        err = b.probe_error(work, b.c5_reference(ref, wd))
+        print(f"{wd} deg cold, omega {om}, {its} iterations: sampled error {err:.3e} m/s")
+        assert err <= 1e-4, (wd, err)
+        for (wd, (om, its)), basis in zip(BASIS.items(), (bx, by)):
             work.set_city_inflow((1, u_ref, 10.0, 0.5, wd))
             work.set_lambda(None)
-            r = work.run(itermax=its, tol=-1.0)
-            assert r["iterations"] == its
+            assert work.run(itermax=its, tol=-1.0, omega=om)["iterations"] == its
             err = b.probe_error(work, b.c5_reference(ref, wd))
-            print(f"{wd} deg cold, {its} iterations: sampled error {err:.3e} m/s")
+            print(f"{wd} deg cold, omega {om}, {its} iterations: sampled error {err:.3e} m/s")
             assert err <= 1e-4, (wd, err)
             Q.lambda_combine(basis, [work], np.array([1.0], np.float32))
         for wd, its in SUPERPOSED.items():
             th = math.radians(270.0 - wd)
             work.set_city_inflow((1, u_ref, 10.0, 0.5, wd))
             Q.lambda_combine(work, [bx, by], np.array([math.cos(th), math.sin(th)], np.float32))
-            work.run(itermax=its, tol=-1.0)
+            work.run(itermax=its, tol=-1.0, omega=1.99)
             err = b.probe_error(work, b.c5_reference(ref, wd))
             print(f"{wd} deg superposed, {its} iterations: sampled error {err:.3e} m/s")
             assert err <= 1e-4, (wd, err)
