@@ -104,6 +104,14 @@ typedef struct qesmc_params {
                         fixes the end at t* + ceil(stop_extend * t*); the
                         iterations up to it run whatever max|dlambda| does.
                         Decided by the device gate, like the reference's.   */
+  int refine_every; /* 0: off.  K > 0 (split path only): the run iterates on a
+                       correction delta of lambda = base + delta, and every K
+                       iterations folds delta into base and forms the
+                       residual of base in FP64 (as qesmc_refine does), so
+                       the fp32 rounding of lambda is not amplified by SOR
+                       into a noise floor of the field, and max|dlambda| (the
+                       stop rule's measure) keeps falling at omega near 2.
+                       In exact arithmetic the iterates are the plain run's. */
 } qesmc_params;
 
 /* qesmc_params.flags: rebuild the stencil encoding from the resident
@@ -135,7 +143,7 @@ int qesmc_destroy(qesmc_ctx *ctx);
 
 /* Reference defaults: omega 1.78f, tol 1e-9f, itermax 500, alpha 1,
  * path AUTO, check_every 64, flags 0, tile fields 0, slab_planes 0,
- * stop_extend 0 (the reference's stop rule). */
+ * stop_extend 0 (the reference's stop rule), refine_every 0. */
 void qesmc_default_params(qesmc_params *p);
 
 const char *qesmc_status_string(int status);
@@ -301,6 +309,22 @@ int qesmc_wall_log_bc(qesmc_ctx *ctx, float z0, double *ms);
  * NULL.  The converged solution does not depend on omega; results at the
  * reference's omega (parity) are unaffected. */
 int qesmc_estimate_omega(qesmc_ctx *ctx, int steps, double *rho, float *omega);
+
+/* One step of mixed-precision iterative refinement of the resident lambda
+ * (no counterpart in the reference, whose fp32 SOR stops where its own
+ * rounding noise does): qesmc_run with the resident lambda as the base of
+ * refine_every (itself when params->refine_every > 0, else no refresh
+ * inside the run).  The residual of the current lambda, r' = R + D lambda -
+ * sum_nb c_nb lambda_nb, is formed in FP64 and replaces R; the same
+ * red-black SOR (omega, tol, itermax, stop rule, tiling) solves for the
+ * correction from zero; lambda += correction (one fp32 rounding); then the
+ * velocity correction and zeroing as in qesmc_run.  Near 2, SOR amplifies
+ * the fp32 rounding of lambda into a noise floor of the velocity field
+ * (about 1e-4 m/s on a 1000 x 1000 x 150 grid at omega 1.99); refinement
+ * takes the field below it.  From lambda = 0 the result is qesmc_run's,
+ * bit for bit.  Split path only (QESMC_PATH_PLAIN or _FUSED:
+ * QESMC_ERR_UNSUPPORTED).  In res, ms_total includes the residual. */
+int qesmc_refine(qesmc_ctx *ctx, const qesmc_params *params, qesmc_result *res);
 
 /* ---- ensembles on one geometry (no counterpart in the reference) ----------
  *

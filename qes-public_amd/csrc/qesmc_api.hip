@@ -85,6 +85,7 @@ void free_geometry(qesmc_ctx *x)
     dfree(x->d_Rc[c]);
     dfree(x->d_mask[c]);
     dfree(x->d_lc2[c]);
+    dfree(x->d_R0[c]);
     for (auto &p : x->d_cc[c]) dfree(p);
   }
   dfree(x->d_canon);
@@ -227,6 +228,7 @@ void qesmc_default_params(qesmc_params *p)
   p->wg_per_cu = 0;
   p->slab_planes = 0;
   p->stop_extend = 0.f;
+  p->refine_every = 0;
 }
 
 int qesmc_create(qesmc_ctx **out, int device)
@@ -366,7 +368,9 @@ int run_prepare(qesmc_ctx *x, const qesmc_params *pp, RunState &st)
   if (p.tile_k < 0 || p.wg_per_cu < 0 || p.slab_planes < -1) return QESMC_ERR_ARG;
   if (!(p.stop_extend >= 0.f && p.stop_extend <= 1000.f)) return QESMC_ERR_ARG;  // also refuses NaN
   if (p.tile_v != 0 && p.tile_v != 1 && p.tile_v != 2 && p.tile_v != 4) return QESMC_ERR_ARG;
+  if (p.refine_every < 0) return QESMC_ERR_ARG;
   st.path = (p.path == QESMC_PATH_PLAIN || p.path == QESMC_PATH_FUSED) ? p.path : QESMC_PATH_SPLIT;
+  if (p.refine_every > 0 && st.path != QESMC_PATH_SPLIT) return QESMC_ERR_UNSUPPORTED;
   HIPCHK(hipSetDevice(x->device));
   if (p.flags & QESMC_RUN_REENCODE) {
     int rc = encode_stencil(x);
@@ -427,6 +431,13 @@ int run_prepare(qesmc_ctx *x, const qesmc_params *pp, RunState &st)
       if (!x->d_lc2[c]) HIPCHK(dalloc(&x->d_lc2[c], x->nsplit));
     if (!x->d_cur) HIPCHK(dalloc(&x->d_cur, 1));
   }
+  if (p.refine_every > 0) {
+    // base lambda and the run's R, besides the correction and the residual
+    for (int c = 0; c < 2; ++c) {
+      if (!x->d_lc2[c]) HIPCHK(dalloc(&x->d_lc2[c], x->nsplit));
+      if (!x->d_R0[c]) HIPCHK(dalloc(&x->d_R0[c], x->nsplit));
+    }
+  }
 
   const long long rows = (long long)p.itermax + 1;
   if (x->err_cap < rows) {
@@ -458,6 +469,7 @@ int run_prepare(qesmc_ctx *x, const qesmc_params *pp, RunState &st)
     HIPCHK(qesmc_launch_set_u32(x, x->d_cur, 0u));
   }
   HIPCHK(qesmc_launch_divergence(x, st.path != QESMC_PATH_PLAIN, p.alpha1));
+  if (p.refine_every > 0) HIPCHK(refine_setup(x, st.masked));
   HIPCHK(hipEventRecord(x->ev[1], s));
   st.first_runs = p.itermax > 0 && (1.0f > p.tol);
   return QESMC_OK;
@@ -516,15 +528,13 @@ int qesmc_run(qesmc_ctx *x, const qesmc_params *pp, qesmc_result *res) { return 
 
 namespace qesmc {
 
-// One resident solve: preparation, the iteration loop with convergence
-// polling one chunk behind the device, and the finish (qesmc_run).
-int run_solve(qesmc_ctx *x, const qesmc_params *pp, qesmc_result *res)
+// The iteration loop of a prepared run, with convergence polling one chunk
+// behind the device; *launched counts the iterations queued.
+int run_loop(qesmc_ctx *x, const RunState &st, int *launched_out)
 {
-  qesmc::RunState st;
-  int rc = qesmc::run_prepare(x, pp, st);
-  if (rc != QESMC_OK) return rc;
   const qesmc_params &p = st.p;
   hipStream_t s = x->stream;
+  int rc = QESMC_OK;
   int launched = 0;
   if (st.first_runs) {
     // Poll convergence one chunk behind the device: after chunk n is queued,
@@ -540,6 +550,11 @@ int run_solve(qesmc_ctx *x, const qesmc_params *pp, qesmc_result *res)
         break;
       }
       ++launched;
+      if (p.refine_every > 0 && (t + 1) % p.refine_every == 0 && t + 1 < p.itermax &&
+          (e = refine_refresh(x, st.masked)) != hipSuccess) {
+        rc = map_hip(e);
+        break;
+      }
       if ((t + 1) % p.check_every == 0 && t + 1 < p.itermax) {
         const int slot = chunk & 1;
         unsigned *dst = x->h_poll + slot * qesmc::kErrSlots;
@@ -565,10 +580,25 @@ int run_solve(qesmc_ctx *x, const qesmc_params *pp, qesmc_result *res)
         ++chunk;
       }
     }
-    if (rc != QESMC_OK) {
-      (void)hipStreamSynchronize(s);
-      return rc;
-    }
+    if (rc != QESMC_OK) (void)hipStreamSynchronize(s);
+  }
+  *launched_out = launched;
+  return rc;
+}
+
+// One resident solve: preparation, the iteration loop and the finish
+// (qesmc_run).
+int run_solve(qesmc_ctx *x, const qesmc_params *pp, qesmc_result *res)
+{
+  qesmc::RunState st;
+  int rc = qesmc::run_prepare(x, pp, st);
+  if (rc != QESMC_OK) return rc;
+  int launched = 0;
+  rc = run_loop(x, st, &launched);
+  if (rc != QESMC_OK) return rc;
+  if (st.p.refine_every > 0) {
+    hipError_t e = refine_fold_final(x);
+    if (e != hipSuccess) return map_hip(e);
   }
   return qesmc::run_finish(x, st, launched, res);
 }

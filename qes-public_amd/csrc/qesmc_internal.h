@@ -126,6 +126,7 @@ struct qesmc_ctx {
   unsigned *d_canon = nullptr; // 6*NZ wall values per layer (float bits)
   float *d_lc2[2] = {};       // second lambda buffer set of the fused path (allocated on first use)
   unsigned *d_cur = nullptr;   // fused path: buffer set holding the latest iterate (0: d_lc, 1: d_lc2)
+  float *d_R0[2] = {};        // refine_every: R of the run (d_Rc then holds the residual of the base)
 
   // convergence slots, kErrSlots words per iteration: row 0 holds the 1.0f
   // sentinel, row t+1 the striped max|dlambda| of iteration t
@@ -157,6 +158,13 @@ struct RunState {
 int run_prepare(qesmc_ctx *x, const qesmc_params *pp, RunState &st);
 int run_finish(qesmc_ctx *x, const RunState &st, int launched, qesmc_result *res);
 int run_solve(qesmc_ctx *x, const qesmc_params *pp, qesmc_result *res);
+int run_loop(qesmc_ctx *x, const RunState &st, int *launched);
+// refine_every: set up base = lambda, R0 = R, R = residual of base, delta
+// = 0 (refine_setup); fold delta into base and re-form the residual
+// (refine_refresh); lambda = base + delta at the end (refine_fold_final).
+hipError_t refine_setup(qesmc_ctx *x, bool masked);
+hipError_t refine_refresh(qesmc_ctx *x, bool masked);
+hipError_t refine_fold_final(qesmc_ctx *x);
 int static_scratch_init(int device);
 float row_max(const unsigned *row);
 

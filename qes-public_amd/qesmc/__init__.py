@@ -41,7 +41,7 @@ EXPORTS = (
     "qesmc_create", "qesmc_destroy", "qesmc_default_params", "qesmc_status_string",
     "qesmc_solve", "qesmc_set_geometry", "qesmc_set_inflow", "qesmc_set_lambda", "qesmc_run",
     "qesmc_get_velocity", "qesmc_get_lambda", "qesmc_divergence_linf", "qesmc_city_build",
-    "qesmc_city_random_buildings", "qesmc_run_static",
+    "qesmc_city_random_buildings", "qesmc_run_static", "qesmc_refine",
 )
 
 
@@ -74,7 +74,7 @@ class Params(C.Structure):
                 ("alpha1", C.c_int), ("alpha2", C.c_int), ("path", C.c_int),
                 ("check_every", C.c_int), ("flags", C.c_int), ("tile_k", C.c_int),
                 ("tile_v", C.c_int), ("wg_per_cu", C.c_int), ("slab_planes", C.c_int),
-                ("stop_extend", C.c_float)]
+                ("stop_extend", C.c_float), ("refine_every", C.c_int)]
 
 
 class Result(C.Structure):
@@ -123,6 +123,7 @@ def lib():
     L.qesmc_set_lambda.argtypes = [P, _fp]
     L.qesmc_run.argtypes = [P, C.POINTER(Params), C.POINTER(Result)]
     L.qesmc_run_static.argtypes = [P, C.POINTER(Params), C.POINTER(Result)]
+    L.qesmc_refine.argtypes = [P, C.POINTER(Params), C.POINTER(Result)]
     L.qesmc_get_velocity.argtypes = [P, _fp, _fp, _fp]
     L.qesmc_get_lambda.argtypes = [P, _fp]
     L.qesmc_divergence_linf.argtypes = [P, C.c_int, C.POINTER(C.c_double)]
@@ -306,6 +307,17 @@ class Context:
         rc = lib().qesmc_run_static(self._h, C.byref(p), C.byref(r))
         if rc != OK and not (allow_nonfinite and rc == ERR_NONFINITE):
             raise QesmcError(rc, "qesmc_run_static")
+        return r.as_dict()
+
+    def refine(self, params: Params | None = None, allow_nonfinite=False, **kw) -> dict:
+        """One mixed-precision refinement of the resident lambda (qesmc_refine):
+        FP64 residual, SOR on the correction from zero, lambda += correction,
+        velocity correction."""
+        p = params if params is not None else default_params(**kw)
+        r = Result()
+        rc = lib().qesmc_refine(self._h, C.byref(p), C.byref(r))
+        if rc != OK and not (allow_nonfinite and rc == ERR_NONFINITE):
+            raise QesmcError(rc, "qesmc_refine")
         return r.as_dict()
 
     def get_velocity(self):
