@@ -776,7 +776,7 @@ def roofline_of(results, lups_per_iteration, workload_key):
     if last["path"] == Q.PATH_PLAIN:
         kernel = "k_rb_plain"
     elif fused:
-        kernel = "k_sor_fused"
+        kernel = "k_sor_march<%d>" % (8 if last["tile_k"] >= 8 else 4)
     else:
         kernel = "k_sor_split<%s, %d>" % ("true" if last["compressed"] else "false", last["tile_v"])
     traffic, source = load_traffic(kernel, workload_key)

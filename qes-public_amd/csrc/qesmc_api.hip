@@ -30,6 +30,7 @@ hipError_t qesmc_launch_div_linf(qesmc_ctx *x, int physical);
 hipError_t qesmc_launch_set_u32(qesmc_ctx *x, unsigned *p, unsigned v);
 hipError_t qesmc_launch_fill_u32(qesmc_ctx *x, unsigned *p, long long count, unsigned v);
 hipError_t qesmc_launch_rows_summary(qesmc_ctx *x, int launched, float tol);
+hipError_t qesmc_fused_prepare(qesmc_ctx *x);
 
 namespace {
 
@@ -89,6 +90,7 @@ void free_geometry(qesmc_ctx *x)
     for (auto &p : x->d_cc[c]) dfree(p);
   }
   dfree(x->d_canon);
+  dfree(x->d_canon8);
   x->have_geom = x->have_inflow = false;
   x->geom_checked = false;
   x->d = qesmc::Dims{};
@@ -422,14 +424,16 @@ int run_prepare(qesmc_ctx *x, const qesmc_params *pp, RunState &st)
     x->kplanes = p.tile_k > 0 ? p.tile_k : 8;
   }
   if (st.path == QESMC_PATH_FUSED) {
-    // one work-group per 124 x 8 x tile_k tile (tile_v and wg_per_cu unused);
-    // 12 planes measured best on 400x400x120 and 1000x1000x150
-    x->kplanes = p.tile_k > 0 ? p.tile_k : 12;
+    // one wave per column tile of tile_k planes (4 or 8; tile_v and
+    // wg_per_cu unused)
+    x->kplanes = p.tile_k > 0 ? p.tile_k : 8;
     x->tile_v = 1;
     x->wg_per_cu = 0;
     for (int c = 0; c < 2; ++c)
       if (!x->d_lc2[c]) HIPCHK(dalloc(&x->d_lc2[c], x->nsplit));
     if (!x->d_cur) HIPCHK(dalloc(&x->d_cur, 1));
+    if (!x->d_canon8) HIPCHK(dalloc(&x->d_canon8, 8LL * x->d.NZ));
+    HIPCHK(qesmc_fused_prepare(x));
   }
   if (p.refine_every > 0) {
     // base lambda and the run's R, besides the correction and the residual

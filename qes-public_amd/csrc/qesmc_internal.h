@@ -124,6 +124,7 @@ struct qesmc_ctx {
   unsigned char *d_mask[2] = {};
   float *d_cc[2][6] = {};     // full coefficients, split (only if not compressible)
   unsigned *d_canon = nullptr; // 6*NZ wall values per layer (float bits)
+  unsigned *d_canon8 = nullptr; // fused path: the same, 8 words per layer (6 used), layer-major
   float *d_lc2[2] = {};       // second lambda buffer set of the fused path (allocated on first use)
   unsigned *d_cur = nullptr;   // fused path: buffer set holding the latest iterate (0: d_lc, 1: d_lc2)
   float *d_R0[2] = {};        // refine_every: R of the run (d_Rc then holds the residual of the base)

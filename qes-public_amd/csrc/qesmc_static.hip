@@ -56,7 +56,8 @@ int qesmc_run_static(qesmc_ctx *x, const qesmc_params *pp, qesmc_result *res)
 {
   if (!x || !pp) return QESMC_ERR_ARG;
   if (pp->itermax > kStaticMaxIter) return QESMC_ERR_LIMIT;
-  if (pp->path == QESMC_PATH_FUSED && (!x->d_lc2[0] || !x->d_lc2[1] || !x->d_cur)) return QESMC_ERR_STATE;
+  if (pp->path == QESMC_PATH_FUSED && (!x->d_lc2[0] || !x->d_lc2[1] || !x->d_cur || !x->d_canon8))
+    return QESMC_ERR_STATE;
   // refine_every allocates its base and R copies on first use
   if (pp->refine_every != 0) return QESMC_ERR_UNSUPPORTED;
   if (hipSetDevice(x->device) != hipSuccess) return QESMC_ERR_NO_DEVICE;
