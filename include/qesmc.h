@@ -272,14 +272,37 @@ int qesmc_get_inputs(qesmc_ctx *ctx, float *e, float *f, float *g, float *h, flo
 
 /* TURBGeneralData::derivativeVelocity (TURBGeneralData.cpp:479-552; device
  * counterpart TURBGeneralData.cu:37-306) on the resident corrected field of
- * the last qesmc_run / qesmc_solve, without the wall log-law corrections
- * (:553-556).  G: NULL (result stays on the device) or 9*(nx-1)(ny-1)(nz-1)
+ * the last qesmc_run / qesmc_solve, without the wall corrections (:553-556,
+ * qesmc_velocity_gradient_walls).  G: NULL (result stays on the device) or 9*(nx-1)(ny-1)(nz-1)
  * floats receiving Gxx, Gxy, Gxz, Gyx, Gyy, Gyz, Gzx, Gzy, Gzz (G_ab = d u_a /
  * d x_b) for the fluid cells with i in [1,nx-3], j in [1,ny-3], k in [1,nz-3],
  * 0 elsewhere.  uniform_z != 0 selects the reference's uniform-grid branch
  * (flagUniformZGrid) with spacing dz; otherwise the stretched-grid formulas
  * on the resident dz_array.  ms (optional): device time of the kernel. */
 int qesmc_velocity_gradient(qesmc_ctx *ctx, int uniform_z, float dz, float *G, double *ms);
+
+/* The wall step of TURBGeneralData::derivativeVelocity (TURBGeneralData.cpp:
+ * 553-556) on the resident gradient of the last qesmc_velocity_gradient call
+ * (QESMC_ERR_STATE without one), for stair-step walls: for buildings (cell
+ * flag 0) and then terrain (flag 2), the velocity wall treatment of
+ * turbParams buildingWallFlag / terrainWallFlag (TURBParams.h:67-89, default
+ * 2; TURBWallBuilding.cpp:37-70, TURBWallTerrain.cpp:37-70):
+ *   1  log law (set_loglaw_stairstep, TURBWallBuilding.cpp:87-169): the
+ *      wall-normal derivatives of the wall-parallel components next to a
+ *      wall are 0.5 (q_f + q_f') / (0.5 d log(0.5 d / z0)); z0 is `z0`
+ *      (WGD->z0) for buildings and, for terrain, the mean of the four
+ *      surrounding entries of z0_domain_u / z0_domain_v (nx*ny floats each,
+ *      WGD->z0_domain_u / _v; read only in this case, else may be NULL);
+ *   2  one-sided differences into the fluid
+ *      (comp_velocity_deriv_finitediff_stairstep, TURBWall.cpp:206-300);
+ *   0  none.
+ * Same cells, pass order and arithmetic as the reference, including its
+ * quirks (the log law's "left wall" pass walks the cells with a wall below;
+ * dy is dx).  Cut cells (flags 7/8, the meshTypeFlag == 1 branch) are not
+ * built: QESMC_ERR_UNSUPPORTED, gradient unchanged.  G, ms: as in
+ * qesmc_velocity_gradient. */
+int qesmc_velocity_gradient_walls(qesmc_ctx *ctx, int building_mode, int terrain_mode, float z0,
+                                  const float *z0_domain_u, const float *z0_domain_v, float *G, double *ms);
 
 /* Wall::wallLogBC (Wall.cpp:352-797) on the resident corrected field, as
  * qesWindsMain.cpp:157-165 applies it after a solve when logLawFlag == 1

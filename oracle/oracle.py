@@ -63,6 +63,8 @@ def lib():
     L.oracle_velocity_gradient.argtypes = [I, I, I, Fl, Fl, Fl, f32p, I, i32p, f32p, f32p, f32p, f32p]
     L.oracle_wall_log_bc.argtypes = [I, I, I, Fl, Fl, Fl, i32p, f32p, f32p, f32p]
     L.oracle_wall_log_bc.restype = C.c_long
+    L.oracle_turb_wall_stairstep.argtypes = [I, I, I, Fl, f32p, Fl, f32p, f32p, I, I, i32p, f32p, f32p, f32p, f32p]
+    L.oracle_turb_wall_stairstep.restype = I
     L.oracle_num_threads.restype = I
     L.oracle_set_num_threads.argtypes = [I]
     _lib = L
@@ -278,6 +280,27 @@ def wall_log_bc(p: Problem, u, v, w, z0, dz=None):
     if n < 0:
         raise ValueError("cut-cell flags (7/8) present: only the stair-step passes are restated")
     return u, v, w, int(n)
+
+
+def turb_wall_stairstep(p: Problem, G, u, v, w, building_mode=2, terrain_mode=2, z0=0.1, z0_domain_u=None,
+                        z0_domain_v=None):
+    """The wall step of TURBGeneralData::derivativeVelocity on stair-step
+    walls (TURBGeneralData.cpp:553-556): buildings (flag 0) then terrain
+    (flag 2), each in its mode (1 log law, 2 one-sided differences, 0 none).
+    Returns a corrected copy of G (9, ncell).  z0_domain_u / z0_domain_v
+    (nx*ny each) are needed only for the terrain log law."""
+    G = np.array(G, np.float32, copy=True)
+    if terrain_mode == 1 and (z0_domain_u is None or z0_domain_v is None):
+        raise ValueError("the terrain log law needs z0_domain_u and z0_domain_v")
+    z0u = np.ascontiguousarray(np.zeros(1) if z0_domain_u is None else z0_domain_u, np.float32)
+    z0v = np.ascontiguousarray(np.zeros(1) if z0_domain_v is None else z0_domain_v, np.float32)
+    rc = lib().oracle_turb_wall_stairstep(p.nx, p.ny, p.nz, p.dx, p.dz_array, float(z0), z0u, z0v,
+                                          int(building_mode), int(terrain_mode), p.icellflag,
+                                          np.ascontiguousarray(u, np.float32), np.ascontiguousarray(v, np.float32),
+                                          np.ascontiguousarray(w, np.float32), G)
+    if rc < 0:
+        raise ValueError("cut-cell flags (7/8) present or a mode outside 0..2")
+    return G
 
 
 def set_num_threads(t: int):

@@ -816,6 +816,142 @@ long oracle_wall_log_bc(int nx, int ny, int nz, float dx, float dz, float z0, co
   return total;
 }
 
+/* ------------------------------------------------------------------------ */
+/* f4: wall corrections of the turbulence velocity gradient (stair-step)    */
+/* ------------------------------------------------------------------------ */
+
+/* The wall step of TURBGeneralData::derivativeVelocity (TURBGeneralData.cpp:
+ * 553-556): for each wall object in order -- buildings (TURBWallBuilding,
+ * cell flag 0) then terrain (TURBWallTerrain, cell flag 2) -- the velocity
+ * wall treatment its flag selects (TURBWallBuilding.cpp:37-70,
+ * TURBWallTerrain.cpp:37-70; turbParams buildingWallFlag / terrainWallFlag,
+ * default 2, TURBParams.h:74):
+ *   mode 1  set_loglaw_stairstep (TURBWallBuilding.cpp:87-169,
+ *           TURBWallTerrain.cpp:88-...): the wall-normal derivatives of the
+ *           wall-parallel components from the log law, 0.5 (q_f + q_f') /
+ *           (0.5 d log(0.5 d / z0)), with d = dz_array[k], dx or dy = dx
+ *           (getDomainSize, Domain.h:104); the building object uses WGD->z0,
+ *           the terrain object the mean of the four surrounding entries of
+ *           z0_domain_u / z0_domain_v; its sixth pass ("left of the cell")
+ *           walks wall_below_indices, as the reference does;
+ *   mode 2  comp_velocity_deriv_finitediff_stairstep (TURBWall.cpp:206-300):
+ *           one-sided differences into the fluid, over z[k+1]-z[k],
+ *           z[k]-z[k-1], dx, dy;
+ *   mode 0  none.
+ * The index lists are TURBWall::get_stairstep_wall_id's (TURBWall.cpp:
+ * 39-121): i in [1,nx-2], j in [1,ny-2], k in [1,nz-3], a cell whose flag is
+ * not 0 or 2, whose neighbour (flat index -(nx-1)(ny-1), +(nx-1)(ny-1), -1,
+ * +1, -(nx-1), +(nx-1): so +1 at i = nx-2 and +(nx-1) at j = ny-2 reach the
+ * next row / plane) has the object's flag, with face(i,j,k).  Passes run in
+ * the order below, above, back, front, right, left, so a later one
+ * overwrites what an earlier one wrote.  Arithmetic as written there: float
+ * sums of two faces, 0.5 (double) times them, the quotient in double, stored
+ * to float.  G holds Gxx..Gzz (9 ncell blocks) and is updated in place. */
+static void turb_wall_object(int nx, int ny, int nz, float dx, const float *dz_array, const float *z, int cf,
+                             int mode, float z0b, const float *z0u, const float *z0v, const int *icellflag,
+                             const float *u, const float *v, const float *w, float *G)
+{
+  if (mode != 1 && mode != 2) return;
+  const long sy = nx - 1, sz = (long)(nx - 1) * (ny - 1), ncell = sz * (nz - 1);
+  float *Gxy = G + ncell, *Gxz = G + 2 * ncell, *Gyx = G + 3 * ncell, *Gyz = G + 5 * ncell;
+  float *Gzx = G + 6 * ncell, *Gzy = G + 7 * ncell;
+  long cnt[6] = {0, 0, 0, 0, 0, 0};
+  long *lc[6], *lf[6];
+  for (int p = 0; p < 6; ++p) {
+    lc[p] = (long *)malloc(sizeof(long) * (size_t)(ncell > 0 ? ncell : 1));
+    lf[p] = (long *)malloc(sizeof(long) * (size_t)(ncell > 0 ? ncell : 1));
+  }
+  for (int i = 1; i < nx - 1; i++)
+    for (int j = 1; j < ny - 1; j++)
+      for (int k = 1; k < nz - 2; k++) {
+        const long c = CELL(i, j, k), f = FACE(i, j, k);
+        if (icellflag[c] == 0 || icellflag[c] == 2) continue;
+        const long nb[6] = {c - sz, c + sz, c - 1, c + 1, c - sy, c + sy};
+        for (int p = 0; p < 6; ++p)
+          if (icellflag[nb[p]] == cf) {
+            lc[p][cnt[p]] = c;
+            lf[p][cnt[p]] = f;
+            cnt[p]++;
+          }
+      }
+  const long fy = nx, fz = (long)nx * ny;
+  const float dy = dx;
+  for (int p = 0; p < 6; ++p) {
+    /* mode 1's sixth pass walks the list of walls below (TURBWallBuilding.cpp:153,
+       TURBWallTerrain.cpp: the same) */
+    const int lp = (mode == 1 && p == 5) ? 0 : p;
+    for (long q = 0; q < cnt[lp]; ++q) {
+      const long c = lc[lp][q], f = lf[lp][q];
+      const int k = (int)(c / sz);
+      const int j = (int)((c - (long)k * sz) / sy);
+      const int i = (int)(c - (long)j * sy - (long)k * sz);
+      float z0 = z0b;
+      if (mode == 1 && cf == 2)
+        z0 = (z0u[i + j * nx] + z0u[i + 1 + j * nx] + z0v[i + j * nx] + z0v[i + (j + 1) * nx]) / 4.0;
+      if (mode == 1) {
+        if (p <= 1) {
+          const double den = 0.5 * dz_array[k] * log(0.5 * dz_array[k] / z0);
+          Gxz[c] = 0.5 * (u[f] + u[f + 1]) / den;
+          Gyz[c] = 0.5 * (v[f] + v[f + fy]) / den;
+        } else if (p <= 3) {
+          const double den = 0.5 * dx * log(0.5 * dx / z0);
+          Gyx[c] = 0.5 * (v[f] + v[f + fy]) / den;
+          Gzx[c] = 0.5 * (w[f] + w[f + fz]) / den;
+        } else {
+          const double den = 0.5 * dy * log(0.5 * dy / z0);
+          Gxy[c] = 0.5 * (u[f] + u[f + 1]) / den;
+          Gzy[c] = 0.5 * (w[f] + w[f + fz]) / den;
+        }
+      } else if (p == 0) { /* wall below: forward difference up */
+        Gxz[c] = (0.5 * (u[f + fz] + u[f + 1 + fz]) - 0.5 * (u[f] + u[f + 1])) / (z[k + 1] - z[k]);
+        Gyz[c] = (0.5 * (v[f + fz] + v[f + fy + fz]) - 0.5 * (v[f] + v[f + fy])) / (z[k + 1] - z[k]);
+      } else if (p == 1) { /* wall above: backward difference */
+        Gxz[c] = (0.5 * (u[f] + u[f + 1]) - 0.5 * (u[f - fz] + u[f + 1 - fz])) / (z[k] - z[k - 1]);
+        Gyz[c] = (0.5 * (v[f] + v[f + fy]) - 0.5 * (v[f - fz] + v[f + fy - fz])) / (z[k] - z[k - 1]);
+      } else if (p == 2) { /* wall in back */
+        Gyx[c] = (0.5 * (v[f + 1] + v[f + 1 + fy]) - 0.5 * (v[f] + v[f + fy])) / dx;
+        Gzx[c] = (0.5 * (w[f + 1] + w[f + 1 + fz]) - 0.5 * (w[f] + w[f + fz])) / dx;
+      } else if (p == 3) { /* wall in front */
+        Gyx[c] = (0.5 * (v[f] + v[f + fy]) - 0.5 * (v[f - 1] + v[f - 1 + fy])) / dx;
+        Gzx[c] = (0.5 * (w[f] + w[f + fz]) - 0.5 * (w[f - 1] + w[f - 1 + fz])) / dx;
+      } else if (p == 4) { /* wall on the right (j-1) */
+        Gxy[c] = (0.5 * (u[f + fy] + u[f + 1 + fy]) - 0.5 * (u[f] + u[f + 1])) / dy;
+        Gzy[c] = (0.5 * (w[f + fy] + w[f + fy + fz]) - 0.5 * (w[f] + w[f + fz])) / dy;
+      } else { /* wall on the left (j+1) */
+        Gxy[c] = (0.5 * (u[f] + u[f + 1]) - 0.5 * (u[f - fy] + u[f + 1 - fy])) / dy;
+        Gzy[c] = (0.5 * (w[f] + w[f + fz]) - 0.5 * (w[f - fy] + w[f - fy + fz])) / dy;
+      }
+    }
+  }
+  for (int p = 0; p < 6; ++p) {
+    free(lc[p]);
+    free(lf[p]);
+  }
+}
+
+/* Returns -1 (nothing done) when a cut cell (flag 7/8) exists or a mode is
+ * not 0, 1, 2; z0_domain_u / z0_domain_v (nx*ny each) are read only when the
+ * terrain mode is 1. */
+int oracle_turb_wall_stairstep(int nx, int ny, int nz, float dx, const float *dz_array, float z0,
+                               const float *z0_domain_u, const float *z0_domain_v, int building_mode,
+                               int terrain_mode, const int *icellflag, const float *u, const float *v, const float *w,
+                               float *G)
+{
+  const long ncell = (long)(nx - 1) * (ny - 1) * (nz - 1);
+  for (long c = 0; c < ncell; ++c)
+    if (icellflag[c] == 7 || icellflag[c] == 8) return -1;
+  if (building_mode < 0 || building_mode > 2 || terrain_mode < 0 || terrain_mode > 2) return -1;
+  float *z_face = (float *)malloc(sizeof(float) * nz), *z = (float *)malloc(sizeof(float) * (nz - 1));
+  oracle_vertical_grid(nz, dz_array, z_face, z);
+  turb_wall_object(nx, ny, nz, dx, dz_array, z, 0, building_mode, z0, z0_domain_u, z0_domain_v, icellflag, u, v,
+                   w, G);
+  turb_wall_object(nx, ny, nz, dx, dz_array, z, 2, terrain_mode, z0, z0_domain_u, z0_domain_v, icellflag, u, v, w,
+                   G);
+  free(z_face);
+  free(z);
+  return 0;
+}
+
 int oracle_num_threads(void)
 {
 #ifdef _OPENMP

@@ -166,6 +166,8 @@ int run_loop(qesmc_ctx *x, const RunState &st, int *launched);
 hipError_t refine_setup(qesmc_ctx *x, bool masked);
 hipError_t refine_refresh(qesmc_ctx *x, bool masked);
 hipError_t refine_fold_final(qesmc_ctx *x);
+// Whether the resident flags hold a cut cell (7 or 8).
+hipError_t find_cut_cells(qesmc_ctx *x, bool *cut);
 int static_scratch_init(int device);
 float row_max(const unsigned *row);
 

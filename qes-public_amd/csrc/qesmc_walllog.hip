@@ -121,6 +121,28 @@ __global__ __launch_bounds__(256) void k_wall_log(Dims d, Pass P, const int *__r
 
 }  // namespace
 
+namespace qesmc {
+
+// Whether the resident flags hold a cut cell (7 or 8), on the context's stream.
+hipError_t find_cut_cells(qesmc_ctx *x, bool *cut)
+{
+  hipStream_t s = x->stream;
+  hipError_t e = hipMemsetAsync(x->d_status, 0, sizeof(unsigned), s);
+  if (e != hipSuccess) return e;
+  long long blocks = (x->ncell + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_find_cut_cells, dim3((unsigned)blocks), dim3(256), 0, s, x->d_flag, x->ncell, x->d_status);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  unsigned *hst = x->h_poll + 2 * qesmc::kErrSlots;
+  if ((e = hipMemcpyAsync(hst, x->d_status, sizeof(unsigned), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+  *cut = (*hst & ST_CUTCELL) != 0;
+  return hipSuccess;
+}
+
+}  // namespace qesmc
+
 extern "C" {
 
 int qesmc_wall_log_bc(qesmc_ctx *x, float z0, double *ms)
@@ -131,16 +153,10 @@ int qesmc_wall_log_bc(qesmc_ctx *x, float z0, double *ms)
   const Dims &d = x->d;
   hipStream_t s = x->stream;
   // cut cells first: their branch of wallLogBC is not built
-  WCHK(hipMemsetAsync(x->d_status, 0, sizeof(unsigned), s));
   {
-    long long blocks = (x->ncell + 255) / 256;
-    if (blocks > 8192) blocks = 8192;
-    hipLaunchKernelGGL(k_find_cut_cells, dim3((unsigned)blocks), dim3(256), 0, s, x->d_flag, x->ncell, x->d_status);
-    WCHK(hipGetLastError());
-    unsigned *hst = x->h_poll + 2 * qesmc::kErrSlots;
-    WCHK(hipMemcpyAsync(hst, x->d_status, sizeof(unsigned), hipMemcpyDeviceToHost, s));
-    WCHK(hipStreamSynchronize(s));
-    if (*hst & ST_CUTCELL) return QESMC_ERR_UNSUPPORTED;
+    bool cut = false;
+    WCHK(qesmc::find_cut_cells(x, &cut));
+    if (cut) return QESMC_ERR_UNSUPPORTED;
   }
   if (d.nz < 4) return QESMC_OK;  // the listed range k in [1, nz-3] is empty
   // Domain dz: the smallest dz_array entry (Domain.cpp:89,131); dy is dx

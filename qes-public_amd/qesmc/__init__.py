@@ -374,6 +374,21 @@ class Context:
                "qesmc_velocity_gradient")
         return G, ms.value
 
+    def velocity_gradient_walls(self, building_mode=2, terrain_mode=2, z0=0.1, z0_domain_u=None, z0_domain_v=None,
+                                copy=True):
+        """The wall step of the gradient on stair-step walls
+        (qesmc_velocity_gradient_walls), on the gradient of the last
+        velocity_gradient call: ((9, ncell) array or None, kernel ms)."""
+        G = np.empty((9, self.city.ncell), np.float32) if copy else None
+        zu = None if z0_domain_u is None else np.ascontiguousarray(z0_domain_u, np.float32)
+        zv = None if z0_domain_v is None else np.ascontiguousarray(z0_domain_v, np.float32)
+        ms = C.c_double()
+        _check(_city_lib().qesmc_velocity_gradient_walls(
+            self._h, int(building_mode), int(terrain_mode), float(z0), _f(zu) if zu is not None else None,
+            _f(zv) if zv is not None else None, G.ctypes.data_as(_fp) if copy else None, C.byref(ms)),
+            "qesmc_velocity_gradient_walls")
+        return G, ms.value
+
     def estimate_omega(self, steps=200):
         """(rho_J, omega) of the resident problem (qesmc_estimate_omega)."""
         rho, om = C.c_double(), C.c_float()
@@ -440,7 +455,8 @@ def autotune(ctx: "Context", iterations=40, params: Params | None = None, **kw) 
 
 EXPORTS = EXPORTS + ("qesmc_set_city", "qesmc_set_city_inflow", "qesmc_get_inputs", "qesmc_lambda_combine",
                      "qesmc_velocity_linf_diff", "qesmc_velocity_gradient", "qesmc_set_probes",
-                     "qesmc_get_probes", "qesmc_wall_log_bc", "qesmc_estimate_omega")
+                     "qesmc_get_probes", "qesmc_wall_log_bc", "qesmc_estimate_omega",
+                     "qesmc_velocity_gradient_walls")
 
 
 class CityShape:
@@ -472,6 +488,9 @@ def _city_lib():
         L.qesmc_velocity_linf_diff.restype = C.c_int
         L.qesmc_velocity_gradient.argtypes = [C.c_void_p, C.c_int, C.c_float, _fp, C.POINTER(C.c_double)]
         L.qesmc_velocity_gradient.restype = C.c_int
+        L.qesmc_velocity_gradient_walls.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_float, _fp, _fp, _fp,
+                                                    C.POINTER(C.c_double)]
+        L.qesmc_velocity_gradient_walls.restype = C.c_int
         L.qesmc_set_probes.argtypes = [C.c_void_p, C.c_longlong, C.POINTER(C.c_longlong)]
         L.qesmc_set_probes.restype = C.c_int
         L.qesmc_get_probes.argtypes = [C.c_void_p, _fp, _fp, _fp]

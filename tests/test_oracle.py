@@ -740,3 +740,186 @@ def test_fp64_solution_is_linear_in_the_wind_direction(oracle, wind):
     # and a wrong combination (sign of the second basis flipped) is far off
     bad = np.cos(th) * fields[270.0] - np.sin(th) * fields[180.0]
     assert np.abs(bad - fields[wind]).max() > 1e-2 * scale
+
+
+# ---------------------------------------------------------------------------
+# f4: wall corrections of the turbulence velocity gradient (stair-step)
+# ---------------------------------------------------------------------------
+def _turb_wall_city(O):
+    """A box building on flat ground (flags from the builder), then flags set
+    by hand so that every pass has cells: a block in the air (walls above), a
+    one-cell gap in x and in y between solid columns (walls on both sides),
+    a fluid cell between a roof and the block (walls below and above), and
+    terrain cells (flag 2) next to building cells."""
+    ncx, ncy, ncz, dx = 24, 20, 12, 2.0
+    dz = O.dz_stretched(1.0 * 1.1 ** np.arange(ncz))
+    p = O.build_city(ncx, ncy, dx, dz, [(16.0, 12.0, 10.0, 8.0, 4.5, 0.0, 0.0)], (1, 4.0, 10.0, 0.1, 250.0))
+    NX, NY, NZ = p.nx - 1, p.ny - 1, p.nz - 1
+    f = p.icellflag.reshape(NZ, NY, NX).copy()
+    f[6:8, 3:6, 3:7] = 0          # floating block at k = 6, 7: walls above the cells under it
+    f[4, 3:6, 3:7] = 0            # a slab at k = 4: the cells at k = 5 have walls below and above
+    f[1:4, 10, 14] = 0            # solid columns at i = 14 and i = 16 around a gap at i = 15
+    f[1:4, 10, 16] = 0
+    f[1:4, 14, 4] = 0             # solid columns at j = 14 and j = 16 around a gap at j = 15
+    f[1:4, 16, 4] = 0
+    f[1:3, 2, 18:21] = 2          # terrain cells, and a building cell beside them at i = 17
+    f[1:3, 2, 17] = 0
+    f[1:3, 2, 22] = 0             # a fluid cell at i = 21 between terrain (i = 20) and building (i = 22)
+    q = O.Problem(p.nx, p.ny, p.nz, p.dx, p.dy, p.dz_array, *p.coeffs, f.ravel().astype(np.int32), p.u0, p.v0,
+                  p.w0)
+    return q, f
+
+
+def _turb_lists(f, cf):
+    """TURBWall::get_stairstep_wall_id's six lists (TURBWall.cpp:76-118) as
+    sets of flat cell ids: i in [1,NX-1], j in [1,NY-1], k in [1,NZ-2], flat
+    neighbours -sz, +sz, -1, +1, -NX, +NX."""
+    NZ, NY, NX = f.shape
+    flat = f.ravel()
+    sz = NX * NY
+    out = [set() for _ in range(6)]
+    for k in range(1, NZ - 1):
+        for j in range(1, NY):
+            for i in range(1, NX):
+                c = i + j * NX + k * sz
+                if flat[c] in (0, 2):
+                    continue
+                for p_, off in enumerate((-sz, sz, -1, 1, -NX, NX)):
+                    if flat[c + off] == cf:
+                        out[p_].add(c)
+    return out
+
+
+def test_turb_wall_finite_differences_of_linear_fields(oracle):
+    """Mode 2 (TURBWall.cpp:206-300): one-sided differences of fields linear
+    in x, y and z give the slopes exactly (to float rounding) in every pass's
+    cells, and cells in no list keep the centred gradient."""
+    O = oracle
+    p, f = _turb_wall_city(O)
+    NZ, NY, NX = f.shape
+    nx, ny, nz = p.nx, p.ny, p.nz
+    _, z = O.vertical_grid(p.dz_array)
+    K, J, I = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    zc = z[np.minimum(K, nz - 2)].astype(np.float64)
+    a, al, b, be, c, ga = 0.7, -0.3, 0.45, 0.25, -0.6, 0.35
+    u = (a * zc + al * J * p.dx).astype(np.float32).ravel()
+    v = (b * I * p.dx + be * zc).astype(np.float32).ravel()
+    w = (c * I * p.dx + ga * J * p.dx).astype(np.float32).ravel()
+    G0 = O.velocity_gradient(p, u, v, w)
+    G = O.turb_wall_stairstep(p, G0, u, v, w, 2, 2)
+    want = {1: al, 2: a, 3: b, 5: be, 6: c, 7: ga}   # Gxy, Gxz, Gyx, Gyz, Gzx, Gzy
+    lists = [a_ | b_ for a_, b_ in zip(_turb_lists(f, 0), _turb_lists(f, 2))]
+    touched = {1: lists[4] | lists[5], 2: lists[0] | lists[1], 3: lists[2] | lists[3], 5: lists[0] | lists[1],
+               6: lists[2] | lists[3], 7: lists[4] | lists[5]}
+    for comp, slope in want.items():
+        cells = np.array(sorted(touched[comp]))
+        assert cells.size > 0, comp
+        np.testing.assert_allclose(G[comp][cells], slope, rtol=2e-4, atol=2e-4, err_msg=str(comp))
+        rest = np.setdiff1d(np.arange(p.ncell), cells)
+        assert np.array_equal(G[comp][rest], G0[comp][rest]), comp
+    for comp in (0, 4, 8):
+        assert np.array_equal(G[comp], G0[comp])
+
+
+def test_turb_wall_later_passes_win(oracle):
+    """Mode 2 with quadratic fields, where the forward and backward one-sided
+    differences differ: a cell with walls below and above takes the backward
+    (above) difference, one between back and front walls the front one, one
+    between right and left walls the left one, and a cell with a building
+    behind and terrain in front the terrain object's front pass (terrain
+    after buildings)."""
+    O = oracle
+    p, f = _turb_wall_city(O)
+    NZ, NY, NX = f.shape
+    nx, ny, nz = p.nx, p.ny, p.nz
+    _, z = O.vertical_grid(p.dz_array)
+    K, J, I = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    zc = z[np.minimum(K, nz - 2)].astype(np.float64)
+    dx = float(p.dx)
+    u = (0.3 * zc ** 2).astype(np.float32).ravel()
+    v = (0.02 * (I * dx) ** 2).astype(np.float32).ravel()
+    w = (0.02 * (J * dx) ** 2).astype(np.float32).ravel()
+    G = O.turb_wall_stairstep(p, O.velocity_gradient(p, u, v, w), u, v, w, 2, 2)
+    cell = lambda i, j, k: i + j * NX + k * NX * NY  # noqa: E731
+    Lb, Lt = _turb_lists(f, 0), _turb_lists(f, 2)
+    # between the slab (k = 4) and the block (k = 6): the backward (above) difference
+    sand = sorted(Lb[0] & Lb[1])
+    assert len(sand) == 3 * 4
+    for cs in sand:
+        kk = cs // (NX * NY)
+        backward = 0.3 * (z[kk] + z[kk - 1])
+        forward = 0.3 * (z[kk + 1] + z[kk])
+        assert abs(G[2][cs] - backward) <= 2e-4 * abs(backward) and abs(forward - backward) > 1e-2
+    # between the columns at i = 14 and 16: the front difference
+    cx = cell(15, 10, 2)
+    assert cx in Lb[2] and cx in Lb[3]
+    back = 0.02 * dx * (2 * 15 + 1)     # (v(i+1) - v(i)) / dx
+    front = 0.02 * dx * (2 * 15 - 1)    # (v(i) - v(i-1)) / dx
+    assert abs(G[3][cx] - front) <= 1e-4 * abs(front) and abs(G[3][cx] - back) > 1e-3
+    # between the columns at j = 14 and 16: the left difference
+    cy = cell(4, 15, 2)
+    assert cy in Lb[4] and cy in Lb[5]
+    right = 0.02 * dx * (2 * 15 + 1)
+    left = 0.02 * dx * (2 * 15 - 1)
+    assert abs(G[7][cy] - left) <= 1e-4 * abs(left) and abs(G[7][cy] - right) > 1e-3
+    # terrain behind (i = 20) and a building in front (i = 22) of cell (21, 2, k): the building
+    # object's front pass runs first, the terrain object's back pass after it and wins
+    for k in (1, 2):
+        ct = cell(21, 2, k)
+        assert ct in Lb[3] and ct in Lt[2]
+        tb = 0.02 * dx * (2 * 21 + 1)   # the back (forward) difference
+        assert abs(G[3][ct] - tb) <= 1e-4 * tb
+
+
+def test_turb_wall_log_law_closed_form(oracle):
+    """Mode 1 (TURBWallBuilding.cpp:87-169 and the terrain variant) on uniform
+    fields: each wall-normal derivative is q / (0.5 d ln(0.5 d / z0)) with
+    d = dz_k, dx, dy = dx; the sixth pass ("left of the cell") walks the
+    cells with a wall below, so a cell with a wall only on its left keeps its
+    centred Gxy, and every cell with a wall below gets the y-formula; the
+    terrain object takes z0 from the mean of the four surrounding
+    z0_domain_u / z0_domain_v entries, summed in float and divided by 4.0."""
+    O = oracle
+    p, f = _turb_wall_city(O)
+    NZ, NY, NX = f.shape
+    U, V, W, z0 = 3.0, -2.0, 0.5, 0.1
+    u = np.full(p.nface, U, np.float32)
+    v = np.full(p.nface, V, np.float32)
+    w = np.full(p.nface, W, np.float32)
+    rng = np.random.default_rng(7)
+    z0u = rng.uniform(0.05, 0.2, p.nx * p.ny).astype(np.float32)
+    z0v = rng.uniform(0.05, 0.2, p.nx * p.ny).astype(np.float32)
+    G0 = O.velocity_gradient(p, u, v, w)
+    G = O.turb_wall_stairstep(p, G0, u, v, w, 1, 1, z0, z0u, z0v)
+    dx = np.float64(np.float32(p.dx))
+    dz = p.dz_array.astype(np.float64)
+
+    def ll(q, d, zz):
+        return np.float32(0.5 * np.float64(np.float32(q) + np.float32(q)) / (0.5 * d * np.log(0.5 * d / np.float64(np.float32(zz)))))
+
+    def z0_terrain(c):
+        k, r = divmod(c, NX * NY)
+        j, i = divmod(r, NX)
+        s = np.float32(np.float32(np.float32(z0u[i + j * p.nx] + z0u[i + 1 + j * p.nx]) + z0v[i + j * p.nx])
+                       + z0v[i + (j + 1) * p.nx])
+        return np.float32(np.float64(s) / 4.0)
+
+    Lb, Lt = _turb_lists(f, 0), _turb_lists(f, 2)
+    # the final value of each component comes from the last object whose pass list holds the cell
+    for comp, passes, qv, dfun in ((2, (0, 1), U, lambda c: dz[c // (NX * NY)]),
+                                   (5, (0, 1), V, lambda c: dz[c // (NX * NY)]),
+                                   (3, (2, 3), V, lambda c: dx), (6, (2, 3), W, lambda c: dx),
+                                   (1, (4, 0), U, lambda c: dx), (7, (4, 0), W, lambda c: dx)):
+        cells_b = set().union(*(Lb[q] for q in passes))
+        cells_t = set().union(*(Lt[q] for q in passes))
+        assert cells_b, comp
+        for c in cells_b | cells_t:
+            want = ll(qv, dfun(c), z0_terrain(c) if c in cells_t else z0)
+            assert G[comp][c] == want, (comp, c, G[comp][c], want)
+        rest = np.setdiff1d(np.arange(p.ncell), np.array(sorted(cells_b | cells_t)))
+        assert np.array_equal(G[comp][rest], G0[comp][rest]), comp
+    # the quirk, explicitly: a wall on the left only leaves Gxy alone
+    left_only = (Lb[5] | Lt[5]) - (Lb[4] | Lt[4] | Lb[0] | Lt[0])
+    assert left_only
+    for c in left_only:
+        assert G[1][c] == G0[1][c]
