@@ -111,7 +111,9 @@ typedef struct qesmc_params {
                        the fp32 rounding of lambda is not amplified by SOR
                        into a noise floor of the field, and max|dlambda| (the
                        stop rule's measure) keeps falling at omega near 2.
-                       In exact arithmetic the iterates are the plain run's. */
+                       In exact arithmetic the iterates are the plain run's.
+                       Not in qesmc_run_batch or qesmc_run_static
+                       (QESMC_ERR_UNSUPPORTED). */
 } qesmc_params;
 
 /* qesmc_params.flags: rebuild the stencil encoding from the resident

@@ -380,6 +380,8 @@ int run_prepare(qesmc_ctx *x, const qesmc_params *pp, RunState &st)
   }
   st.masked = x->compressible;
   if (st.path == QESMC_PATH_FUSED && !st.masked) return QESMC_ERR_UNSUPPORTED;
+  // the fused march addresses its arrays with 32-bit element offsets
+  if (st.path == QESMC_PATH_FUSED && x->nsplit >= (1LL << 31)) return QESMC_ERR_UNSUPPORTED;
   x->kplanes = p.tile_k > 0 ? p.tile_k : x->kplanes_auto;
   x->tile_v = p.tile_v > 0 ? p.tile_v : x->tile_v_auto;
   if (x->kplanes == 0 || x->tile_v == 0) {

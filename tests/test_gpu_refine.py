@@ -191,3 +191,6 @@ def test_refine_every_refusals(qesmc, oracle):
         with pytest.raises(qesmc.QesmcError) as e:
             ctx.run_static(itermax=10, refine_every=5)
         assert e.value.code == qesmc.ERR_UNSUPPORTED
+        with pytest.raises(qesmc.QesmcError) as e:
+            qesmc.run_batch([ctx], itermax=10, refine_every=5)
+        assert e.value.code == qesmc.ERR_UNSUPPORTED
