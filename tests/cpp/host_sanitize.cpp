@@ -62,6 +62,10 @@ long oracle_wall_log_bc(int nx, int ny, int nz, float dx, float dz, float z0, co
 void oracle_velocity_gradient(int nx, int ny, int nz, float dx, float dy, float dz, const float *dz_array,
                               int uniform_z, const int *icellflag, const float *u, const float *v, const float *w,
                               float *G);
+int oracle_turb_wall_stairstep(int nx, int ny, int nz, float dx, const float *dz_array, float z0,
+                               const float *z0_domain_u, const float *z0_domain_v, int building_mode,
+                               int terrain_mode, const int *icellflag, const float *u, const float *v, const float *w,
+                               float *G);
 }
 
 static int failures = 0;
@@ -189,6 +193,17 @@ int main(int argc, char **argv)
     // every wall face, including the wrapped ones at the domain edges)
     std::vector<float> lu = u, lv = v, lw = w;
     EXPECT(oracle_wall_log_bc(nx, ny, nz, dx, 1.0f, 0.1f, flag.data(), lu.data(), lv.data(), lw.data()) > 0);
+  }
+  {
+    // the gradient's wall step in every mode pair (faces at +-1, +-nx, +-nx*ny
+    // of the wall cells, z0 arrays at the four surrounding entries)
+    std::vector<float> z0u((size_t)nx * ny, 0.1f), z0v((size_t)nx * ny, 0.12f);
+    for (int bm = 0; bm < 3; ++bm)
+      for (int tm = 0; tm < 3; ++tm) {
+        std::vector<float> Gw = G;
+        EXPECT(oracle_turb_wall_stairstep(nx, ny, nz, dx, dz.data(), 0.1f, z0u.data(), z0v.data(), bm, tm,
+                                          flag.data(), u.data(), v.data(), w.data(), Gw.data()) == 0);
+      }
   }
 
   // ---- 2. a valid workspace file round trip --------------------------------
