@@ -26,9 +26,10 @@
 // lambda read and write, R and the mask: 13 bytes, against 2 x 17 per cell
 // pair for two split colour passes, plus the halo re-reads (K+4 black planes
 // loaded per K relaxed, W+2 lanes per W, L+2 red rows per L).  As built it
-// is bound by latency, not by bytes: with 161 registers per lane (3 waves
-// per SIMD) each step waits for its own loads, and the halo relaxations
-// add a third to the arithmetic, so the split kernel stays the default.
+// is bound by instruction issue, not by bytes: the halo relaxations, lane
+// shuffles and the register rotation of the march about double the split
+// kernel's arithmetic per update (loading the next step during the current
+// one changed nothing), so the split kernel stays the default.
 #include "qesmc_sor_tile.h"
 
 namespace qesmc {
