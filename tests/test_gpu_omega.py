@@ -108,3 +108,34 @@ def test_estimated_omega_reaches_the_field_criterion_sooner(qesmc, oracle):
             print(seed, om, its)
             assert its[om] is not None and its[1.78] is not None
             assert its[om] * 2 <= its[1.78]
+
+
+def test_estimate_refusals_and_small_domains(qesmc, oracle):
+    """Arguments and state: steps in [1, 100000], geometry resident; on a
+    domain with a single relaxed cell the estimate is that cell's own
+    coupling (its k = 0 mirror term over its diagonal), and the one-step
+    estimate is a Ritz value, never above the spectral radius."""
+    with qesmc.Context(0) as ctx:
+        with pytest.raises(qesmc.QesmcError) as e:
+            ctx.estimate_omega(steps=10)
+        assert e.value.code == qesmc.ERR_STATE
+        p = oracle.build_city(40, 30, 1.0, oracle.dz_uniform(2, 1.0), [], (1, 4.0, 10.0, 0.5, 270.0))
+        ctx.set_geometry(p)
+        for bad in (0, 100001):
+            with pytest.raises(qesmc.QesmcError) as e:
+                ctx.estimate_omega(steps=bad)
+            assert e.value.code == qesmc.ERR_ARG
+        r1, _ = ctx.estimate_omega(steps=1)
+        r, om = ctx.estimate_omega(steps=400)
+        assert 0.0 < r1 <= r + 1e-12 < 1.0
+        assert 1.0 < om < 2.0
+        # one relaxed cell (i = j = k = 1 of a 3 x 3 x 3-cell box) with a
+        # coupling n = 0.5 to the ground ghost below it, which the mirror makes
+        # its own value: rho_J = n / D
+        q = oracle.build_city(3, 3, 1.0, oracle.dz_uniform(2, 1.0), [], (1, 4.0, 10.0, 0.5, 270.0))
+        c = 1 + 1 * 3 + 1 * 9
+        q.coeffs[5][c] = 0.5
+        ctx.set_geometry(q)
+        r, _ = ctx.estimate_omega(steps=5)
+        co = [float(a[c]) for a in q.coeffs]
+        assert co[5] == 0.5 and abs(r - 0.5 / sum(co)) <= 1e-12, (r, co)
