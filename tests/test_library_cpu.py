@@ -135,3 +135,33 @@ def test_resident_calls_without_context_are_rejected(qesmc):
     assert L.qesmc_run(None, None, None) == qesmc.ERR_ARG
     assert L.qesmc_set_lambda(None, None) == qesmc.ERR_ARG
     assert L.qesmc_destroy(None) == qesmc.OK
+
+
+def test_c_program_links_every_entry_point(qesmc, tmp_path):
+    """A C99 program that includes the header and takes the address of every
+    function it declares links against libqesmc.so (each declaration has a
+    C-linkage definition of that name), and the entry points refuse a NULL
+    context with QESMC_ERR_ARG without touching a device."""
+    import re
+    hdr = open(os.path.join(ROOT, "include", "qesmc.h")).read()
+    names = sorted(set(re.findall(r"^\s*(?:int|void|const char \*)\s*(qesmc_\w+)\s*\(", hdr, re.M)))
+    assert "qesmc_refine" in names and "qesmc_velocity_gradient_walls" in names and len(names) > 30
+    src = ["#include <stdio.h>", '#include "qesmc.h"', "int main(void){", "int bad = 0;"]
+    src += [f"printf(\"%d\\n\", (int)((void *)&{n} != (void *)0));" for n in names]
+    src += ["qesmc_params p; qesmc_default_params(&p); qesmc_result r;",
+            "bad |= qesmc_refine(0, &p, &r) != QESMC_ERR_ARG;",
+            "bad |= qesmc_run_static(0, &p, &r) != QESMC_ERR_ARG;",
+            "double rho; float om;",
+            "bad |= qesmc_estimate_omega(0, 10, &rho, &om) != QESMC_ERR_ARG;",
+            "bad |= qesmc_velocity_gradient_walls(0, 2, 2, 0.1f, 0, 0, 0, 0) != QESMC_ERR_ARG;",
+            "bad |= qesmc_wall_log_bc(0, 0.1f, 0) != QESMC_ERR_ARG;",
+            "printf(bad ? \"FAIL\\n\" : \"OK\\n\"); return bad;}"]
+    c = tmp_path / "links.c"
+    c.write_text("\n".join(src))
+    exe = tmp_path / "links"
+    libdir = os.path.join(ROOT, "qes-public_amd", "qesmc")
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-I", os.path.join(ROOT, "include"), str(c), "-o", str(exe),
+                    "-L", libdir, "-lqesmc", f"-Wl,-rpath,{libdir}"], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert out.stdout.strip().splitlines()[-1] == "OK"
