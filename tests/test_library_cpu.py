@@ -142,7 +142,6 @@ def test_c_program_links_every_entry_point(qesmc, tmp_path):
     function it declares links against libqesmc.so (each declaration has a
     C-linkage definition of that name), and the entry points refuse a NULL
     context with QESMC_ERR_ARG without touching a device."""
-    import re
     hdr = open(os.path.join(ROOT, "include", "qesmc.h")).read()
     names = sorted(set(re.findall(r"^\s*(?:int|void|const char \*)\s*(qesmc_\w+)\s*\(", hdr, re.M)))
     assert "qesmc_refine" in names and "qesmc_velocity_gradient_walls" in names and len(names) > 30
