@@ -362,14 +362,15 @@ int qesmc_run_batch(qesmc_ctx **ctxs, int n, const qesmc_params *params, qesmc_r
   bool all_masked = true;
   for (int i = 0; i < n; ++i) all_masked = all_masked && st[i].masked;
   // persistent grid: never more work-groups than can be resident at once.
-  // Unless the caller sets it: 5 per CU when the batch has >= 48 wave tiles
-  // per CU (measured 5-8 % faster than 4 on batches of 74-76 tiles per CU,
-  // one 400 x 400 x 120 city, two 280 x 280 x 120 or four 200 x 200 x 120,
-  // and on the 64-city batch), else the single-problem kernel's choice, 4 for
-  // two half-cells per lane and 5 for one (one 256 x 256 x 96 city, 25 tiles
-  // per CU: 5 is 7 % slower than 4).  Members' autotuned values tune the
-  // single-problem kernel and are not used here.
-  int wg = p.wg_per_cu > 0 ? p.wg_per_cu : total >= 48LL * lead->n_cu ? 5 : vq == 2 ? 4 : 5;
+  // Unless the caller sets it: 5 per CU when the batch has >= 55 wave tiles
+  // per CU (measured 3 % faster than 4 at 61 tiles per CU and 5-8 % at
+  // 74-76: one 400 x 400 x 120 city, two 280 x 280 x 120 or four
+  // 200 x 200 x 120, and on the 64-city batch), else the single-problem
+  // kernel's choice, 4 for two half-cells per lane and 5 for one (4 measured
+  // faster at 25, 32, 39 and 49 tiles per CU, by 7 %, 0.6 %, 8 % and 4 %).
+  // Members' autotuned values tune the single-problem kernel and are not
+  // used here.
+  int wg = p.wg_per_cu > 0 ? p.wg_per_cu : total >= 55LL * lead->n_cu ? 5 : vq == 2 ? 4 : 5;
   {
     int fit = 0;
     const hipError_t oe =
