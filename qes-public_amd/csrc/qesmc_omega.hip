@@ -119,11 +119,48 @@ __global__ __launch_bounds__(256) void k_omega_apply(Dims d, const float *__rest
   }
 }
 
-// The next Lanczos vector: v' = (w - alpha v - beta u) * inv, u' = v.
-__global__ __launch_bounds__(256) void k_omega_next(long long total, double alpha, double beta, double inv,
+// The step's Lanczos coefficients from the partial sums of k_omega_apply,
+// summed in a fixed order by one work-group: hist[2 it] = alpha = w.v,
+// hist[2 it + 1] = beta' = sqrt(|w|^2 - alpha^2 - beta^2) (v, u orthonormal,
+// u.w = beta, the previous beta'), or 0 when that is not positive (the
+// Krylov space is exhausted; the host stops there).
+__global__ __launch_bounds__(256) void k_omega_scalars(const double *__restrict__ partial, int it,
+                                                       double *__restrict__ hist)
+{
+  __shared__ double red[2][256];
+  double s0 = 0.0, s1 = 0.0;
+  for (int q = 0; q < kDotBlocks / 256; ++q) {
+    s0 += partial[threadIdx.x * (kDotBlocks / 256) + q];
+    s1 += partial[kDotBlocks + threadIdx.x * (kDotBlocks / 256) + q];
+  }
+  red[0][threadIdx.x] = s0;
+  red[1][threadIdx.x] = s1;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + h];
+      red[1][threadIdx.x] += red[1][threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double alpha = red[0][0];
+    const double beta = it > 0 ? hist[2 * it - 1] : 0.0;
+    const double b2 = red[1][0] - alpha * alpha - beta * beta;
+    hist[2 * it] = alpha;
+    hist[2 * it + 1] = b2 > 0.0 ? sqrt(b2) : 0.0;
+  }
+}
+
+// The next Lanczos vector: v' = (w - alpha v - beta u) / beta', u' = v, with
+// the coefficients of step `it` from hist (nothing when beta' is 0).
+__global__ __launch_bounds__(256) void k_omega_next(long long total, int it, const double *__restrict__ hist,
                                                     double *__restrict__ u, double *__restrict__ v,
                                                     const double *__restrict__ w)
 {
+  const double alpha = hist[2 * it], beta = it > 0 ? hist[2 * it - 1] : 0.0, nb = hist[2 * it + 1];
+  if (!(nb > 0.0)) return;
+  const double inv = 1.0 / nb;
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long c = blockIdx.x * 256LL + threadIdx.x; c < total; c += stride) {
     const double vc = v[c];
@@ -239,31 +276,38 @@ int qesmc_estimate_omega(qesmc_ctx *x, int steps, double *rho, float *omega)
   hipLaunchKernelGGL(k_omega_scale, grid, block, 0, st, total, 1.0 / std::sqrt(nrm2), v);
   OCHK(hipGetLastError());
   // Lanczos with one reduction per step: w = M v gives alpha = w.v and
-  // |w|^2 together, and the next beta from |w - alpha v - beta u|^2 =
-  // |w|^2 - alpha^2 - beta^2 (v, u orthonormal, u.w = beta)
-  std::vector<double> a, b;
-  double beta = 0.0, last = -1.0, r = 0.0;
+  // |w|^2 together, the next beta from |w - alpha v - beta u|^2 =
+  // |w|^2 - alpha^2 - beta^2.  The coefficients stay on the device; the host
+  // reads them every 10 steps to test the largest Ritz value for
+  // convergence.
+  Buf bh;
+  OCHK(hipMalloc(&bh.p, sizeof(double) * 2 * (size_t)steps));
+  double *hist = (double *)bh.p;
+  std::vector<double> hh(2 * (size_t)steps);
+  double last = -1.0, r = 0.0;
   for (int it = 0; it < steps; ++it) {
     hipLaunchKernelGGL(k_omega_apply, grid, block, 0, st, d, cf[0], cf[1], cf[2], cf[3], cf[4], cf[5], x->d_dz, s, v,
                        w, part);
+    hipLaunchKernelGGL(k_omega_scalars, dim3(1), block, 0, st, (const double *)part, it, hist);
+    hipLaunchKernelGGL(k_omega_next, grid, block, 0, st, total, it, (const double *)hist, u, v, (const double *)w);
     OCHK(hipGetLastError());
-    double ab[2] = {0.0, 0.0};
-    OCHK(sums(2, ab));
-    const double alpha = ab[0];
-    a.push_back(alpha);
-    const double b2 = ab[1] - alpha * alpha - beta * beta;
-    // converged Ritz value: the largest eigenvalue of the Lanczos matrix has
-    // stopped moving (checked every 10 steps)
-    if ((it + 1) % 10 == 0 || it + 1 == steps || !(b2 > 0.0)) {
-      r = tridiag_max_eig(a, b);
-      if (std::fabs(r - last) <= 1e-12 || !(b2 > 0.0) || it + 1 == steps) break;
-      last = r;
+    if ((it + 1) % 10 != 0 && it + 1 != steps) continue;
+    OCHK(hipMemcpyAsync(hh.data(), hist, sizeof(double) * 2 * (size_t)(it + 1), hipMemcpyDeviceToHost, st));
+    OCHK(hipStreamSynchronize(st));
+    // the tridiagonal Lanczos matrix up to the first exhausted step
+    std::vector<double> a, b;
+    bool done = false;
+    for (int q = 0; q <= it; ++q) {
+      a.push_back(hh[2 * (size_t)q]);
+      if (!(hh[2 * (size_t)q + 1] > 0.0)) {
+        done = true;
+        break;
+      }
+      if (q < it) b.push_back(hh[2 * (size_t)q + 1]);
     }
-    const double nb = std::sqrt(b2);
-    b.push_back(nb);
-    hipLaunchKernelGGL(k_omega_next, grid, block, 0, st, total, alpha, beta, 1.0 / nb, u, v, (const double *)w);
-    OCHK(hipGetLastError());
-    beta = nb;
+    r = tridiag_max_eig(a, b);
+    if (done || std::fabs(r - last) <= 1e-12 || it + 1 == steps) break;
+    last = r;
   }
   if (rho) *rho = r;
   if (omega) {
