@@ -23,7 +23,7 @@
 // The estimate.  Lanczos on M in FP64 from a smooth start (the product of
 // the lowest Dirichlet modes in x and y and a vertical half cosine, which is
 // close to the dominant eigenvector, so few steps resolve it), dot products
-// reduced per work-group and summed on the host in a fixed order, then the
+// reduced per work-group and the partials summed in a fixed order, then the
 // largest eigenvalue of the tridiagonal Lanczos matrix by bisection on its
 // Sturm sequence.  The Ritz value never exceeds lambda_max.
 #include <cmath>
@@ -41,19 +41,28 @@ using qesmc::Dims;
     if (_e != hipSuccess) return qesmc::map_hip_error(_e); \
   } while (0)
 
-constexpr int kDotBlocks = 1024;
+constexpr int kDotBlocks = 2048;  // a multiple of kXcds
+constexpr int kXcds = 8;
 
 __device__ inline bool active_cell(const Dims &d, int i, int j, int k, int flag)
 {
   return i >= 1 && i <= d.NX - 2 && j >= 1 && j <= d.NY - 2 && k >= 1 && k <= d.NZ - 2 && flag != 0 && flag != 2;
 }
 
-// s[c] = sqrt(D_c W_c) on active cells, 0 elsewhere; v[c] = the smooth start.
+// The iteration runs in the coordinates y = S^-1 v, S = (D W)^1/2, so that the
+// operator needs no square root or division per neighbour: with
+// q = W C y, M v = S^-1 q, v.(M v) = q.y and |M v|^2 = q.z for z = q / (D W)
+// = S^-1 (M v), and the Lanczos recurrence on v is the same recurrence on y
+// (divided through by S).
+
+// inv[c] = 1 / (D_c W_c) on active cells, 0 elsewhere; y[c] = the smooth
+// start (the lowest Jacobi mode's shape, which in these coordinates is the
+// dominant eigenvector's direction).
 __global__ __launch_bounds__(256) void k_omega_init(Dims d, const float *__restrict__ e, const float *__restrict__ f,
                                                     const float *__restrict__ g, const float *__restrict__ h,
                                                     const float *__restrict__ m, const float *__restrict__ n,
                                                     const int *__restrict__ flag, const float *__restrict__ dz,
-                                                    double *__restrict__ s, double *__restrict__ v)
+                                                    double *__restrict__ inv, double *__restrict__ y)
 {
   const long long total = (long long)d.NX * d.NY * d.NZ;
   const long long stride = (long long)gridDim.x * blockDim.x;
@@ -64,44 +73,81 @@ __global__ __launch_bounds__(256) void k_omega_init(Dims d, const float *__restr
     const int j = (int)(p / d.NX);
     const int i = (int)(p - (long long)j * d.NX);
     if (!active_cell(d, i, j, k, flag[c])) {
-      s[c] = 0.0;
-      v[c] = 0.0;
+      inv[c] = 0.0;
+      y[c] = 0.0;
       continue;
     }
     const double D = (double)e[c] + f[c] + g[c] + h[c] + m[c] + n[c];
-    s[c] = sqrt(D * (double)dz[k]);
-    v[c] = sin(pi * i / (d.NX - 1)) * sin(pi * j / (d.NY - 1)) * cos(0.5 * pi * (k - 0.5) / (d.NZ - 1.5));
+    inv[c] = 1.0 / (D * (double)dz[k]);
+    y[c] = sin(pi * i / (d.NX - 1)) * sin(pi * j / (d.NY - 1)) * cos(0.5 * pi * (k - 0.5) / (d.NZ - 1.5));
   }
 }
 
-// w = M v on active cells (0 elsewhere); partial[b] / partial[kDotBlocks + b]
-// = this block's sums of w.v and w.w.
+// z = S^-1 M S y on active cells; partial[b] / partial[kDotBlocks + b] = this
+// block's sums of v.(M v) and |M v|^2.  Work-groups take the interior rows
+// (j, k) in turn and their threads the row's interior cells; every other
+// cell of z is left as it is (0: the buffers start at 0 off the active set
+// and the recurrence keeps them there).  With MASKED the coefficients come
+// from the wall mask (split layout: cell (i,j,k) at half-index i/2 of row
+// (j,k) of colour (i+j+k)&1) and the per-layer wall values, which give the
+// resident coefficients bit for bit on a mask-encoded geometry and read 1
+// byte per cell instead of 24; otherwise from the six coefficient arrays.
+template <bool MASKED>
 __global__ __launch_bounds__(256) void k_omega_apply(Dims d, const float *__restrict__ e, const float *__restrict__ f,
                                                      const float *__restrict__ g, const float *__restrict__ h,
                                                      const float *__restrict__ m, const float *__restrict__ n,
-                                                     const float *__restrict__ dz, const double *__restrict__ s,
-                                                     const double *__restrict__ v, double *__restrict__ w,
+                                                     const unsigned char *__restrict__ mask0,
+                                                     const unsigned char *__restrict__ mask1,
+                                                     const unsigned *__restrict__ canon,
+                                                     const float *__restrict__ dz, const double *__restrict__ inv,
+                                                     const double *__restrict__ y, double *__restrict__ z,
                                                      double *__restrict__ partial)
 {
-  const long long total = (long long)d.NX * d.NY * d.NZ;
-  const long long stride = (long long)gridDim.x * blockDim.x;
+  const int ny = d.NY - 2, rows = ny > 0 && d.NZ > 2 ? ny * (d.NZ - 2) : 0;
+  // the rows split into one contiguous range per XCD (work-groups go to the
+  // XCDs in turn), so the neighbouring rows a work-group reads are in its
+  // own XCD's L2
+  const int xcd = (int)blockIdx.x % kXcds, per = (int)gridDim.x / kXcds;
+  const int span = (rows + kXcds - 1) / kXcds, r0 = xcd * span, r1 = min(rows, r0 + span);
   double acc = 0.0, acc2 = 0.0;
-  for (long long c = blockIdx.x * 256LL + threadIdx.x; c < total; c += stride) {
-    const double sc = s[c];
-    if (sc == 0.0) {
-      w[c] = 0.0;
-      continue;
+  for (int r = r0 + (int)blockIdx.x / kXcds; r < r1; r += per) {
+    const int k = 1 + r / ny, j = 1 + r % ny;
+    const long long row = (long long)k * d.NY + j;
+    const double dzk = (double)dz[k];
+    double canon_k[6];
+    if (MASKED) {
+#pragma unroll
+      for (int a = 0; a < 6; ++a) canon_k[a] = (double)__uint_as_float(canon[a * d.NZ + k]);
     }
-    const int k = (int)(c / d.sz);
-    // y = v / s on the neighbours (0 off the active set, where s = 0 and v = 0)
-    auto y = [&](long long q) { return s[q] != 0.0 ? v[q] / s[q] : 0.0; };
-    double cy = (double)e[c] * y(c + 1) + (double)f[c] * y(c - 1) + (double)g[c] * y(c + d.sy) +
-                (double)h[c] * y(c - d.sy) + (double)m[c] * y(c + d.sz);
-    cy += (k == 1) ? (double)n[c] * y(c) : (double)n[c] * y(c - d.sz);  // the k = 0 mirror
-    const double wc = (double)dz[k] * cy / sc;
-    w[c] = wc;
-    acc += wc * v[c];
-    acc2 += wc * wc;
+    for (int i = 1 + (int)threadIdx.x; i <= d.NX - 2; i += 256) {
+      const long long c = row * d.NX + i;
+      const double ic = inv[c];
+      if (ic == 0.0) continue;
+      double cf[6];
+      if (MASKED) {
+        const long long x = row * d.HP + (i >> 1);
+        const unsigned mk = ((i + j + k) & 1) ? mask1[x] : mask0[x];
+#pragma unroll
+        for (int a = 0; a < 6; ++a) cf[a] = ((mk >> a) & 1u) ? canon_k[a] : 0.0;
+      } else {
+        cf[0] = e[c];
+        cf[1] = f[c];
+        cf[2] = g[c];
+        cf[3] = h[c];
+        cf[4] = m[c];
+        cf[5] = n[c];
+      }
+      // y is 0 off the active set, so those neighbours drop out by themselves
+      const double yc = y[c];
+      double cy =
+          cf[0] * y[c + 1] + cf[1] * y[c - 1] + cf[2] * y[c + d.sy] + cf[3] * y[c - d.sy] + cf[4] * y[c + d.sz];
+      cy += cf[5] * ((k == 1) ? yc : y[c - d.sz]);  // the k = 0 mirror
+      const double q = dzk * cy;
+      const double zc = q * ic;
+      z[c] = zc;
+      acc += q * yc;
+      acc2 += q * zc;
+    }
   }
   for (int off = 32; off > 0; off >>= 1) {
     acc += __shfl_xor(acc, off, 64);
@@ -152,20 +198,25 @@ __global__ __launch_bounds__(256) void k_omega_scalars(const double *__restrict_
   }
 }
 
-// The next Lanczos vector: v' = (w - alpha v - beta u) / beta', u' = v, with
-// the coefficients of step `it` from hist (nothing when beta' is 0).
+// The next Lanczos vector, written over z: z <- (z - alpha y - beta yp) /
+// beta' (the host then rotates yp <- y, y <- z), with the coefficients of
+// step `it` from hist (nothing when beta' is 0).  Two cells per thread; the
+// thread after the last pair takes an odd count's last cell.
 __global__ __launch_bounds__(256) void k_omega_next(long long total, int it, const double *__restrict__ hist,
-                                                    double *__restrict__ u, double *__restrict__ v,
-                                                    const double *__restrict__ w)
+                                                    const double *__restrict__ yp, const double *__restrict__ y,
+                                                    double *__restrict__ z)
 {
   const double alpha = hist[2 * it], beta = it > 0 ? hist[2 * it - 1] : 0.0, nb = hist[2 * it + 1];
   if (!(nb > 0.0)) return;
-  const double inv = 1.0 / nb;
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long c = blockIdx.x * 256LL + threadIdx.x; c < total; c += stride) {
-    const double vc = v[c];
-    v[c] = (w[c] - alpha * vc - beta * u[c]) * inv;
-    u[c] = vc;
+  const double inb = 1.0 / nb;
+  const long long t = blockIdx.x * 256LL + threadIdx.x, pairs = total / 2;
+  if (t < pairs) {
+    const double2 a = reinterpret_cast<const double2 *>(z)[t], b = reinterpret_cast<const double2 *>(y)[t],
+                  c = reinterpret_cast<const double2 *>(yp)[t];
+    reinterpret_cast<double2 *>(z)[t] =
+        make_double2((a.x - alpha * b.x - beta * c.x) * inb, (a.y - alpha * b.y - beta * c.y) * inb);
+  } else if (t == pairs && (total & 1)) {
+    z[total - 1] = (z[total - 1] - alpha * y[total - 1] - beta * yp[total - 1]) * inb;
   }
 }
 
@@ -176,13 +227,14 @@ __global__ __launch_bounds__(256) void k_omega_scale(long long total, double inv
   for (long long c = blockIdx.x * 256LL + threadIdx.x; c < total; c += stride) v[c] *= inv;
 }
 
-// sum of squares partials of v.
-__global__ __launch_bounds__(256) void k_omega_norm(long long total, const double *__restrict__ v,
-                                                    double *__restrict__ partial)
+// partials of |v|^2 = sum y^2 D W.
+__global__ __launch_bounds__(256) void k_omega_norm(long long total, const double *__restrict__ inv,
+                                                    const double *__restrict__ y, double *__restrict__ partial)
 {
   const long long stride = (long long)gridDim.x * blockDim.x;
   double acc = 0.0;
-  for (long long c = blockIdx.x * 256LL + threadIdx.x; c < total; c += stride) acc += v[c] * v[c];
+  for (long long c = blockIdx.x * 256LL + threadIdx.x; c < total; c += stride)
+    if (inv[c] != 0.0) acc += y[c] * y[c] / inv[c];
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
   __shared__ double red[4];
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
@@ -248,7 +300,10 @@ int qesmc_estimate_omega(qesmc_ctx *x, int steps, double *rho, float *omega)
   OCHK(hipMalloc(&bu.p, sizeof(double) * (size_t)total));
   OCHK(hipMalloc(&bw.p, sizeof(double) * (size_t)total));
   OCHK(hipMalloc(&bp.p, 2 * sizeof(double) * kDotBlocks));
-  double *s = (double *)bs.p, *v = (double *)bv.p, *u = (double *)bu.p, *w = (double *)bw.p, *part = (double *)bp.p;
+  // inv = 1 / (D W); y, the previous vector yp and z rotate through three
+  // buffers
+  double *inv = (double *)bs.p, *y = (double *)bv.p, *yp = (double *)bu.p, *z = (double *)bw.p,
+         *part = (double *)bp.p;
   std::vector<double> hp(2 * kDotBlocks);
   // the partial sums of one or two block reductions, summed on the host in a
   // fixed order, so the estimate is deterministic
@@ -262,18 +317,22 @@ int qesmc_estimate_omega(qesmc_ctx *x, int steps, double *rho, float *omega)
     }
     return e;
   };
-  const dim3 grid(kDotBlocks), block(256);
+  const dim3 grid(kDotBlocks), block(256), next_grid((unsigned)((total / 2 + 256) / 256));
   float *const *cf = x->d_coef;
+  // the wall-mask encoding built when the geometry was committed, when the
+  // coefficients have one
+  const bool masked = x->compressible;
   hipLaunchKernelGGL(k_omega_init, grid, block, 0, st, d, cf[0], cf[1], cf[2], cf[3], cf[4], cf[5], x->d_flag,
-                     x->d_dz, s, v);
+                     x->d_dz, inv, y);
   OCHK(hipGetLastError());
-  OCHK(hipMemsetAsync(u, 0, sizeof(double) * (size_t)total, st));
-  hipLaunchKernelGGL(k_omega_norm, grid, block, 0, st, total, v, part);
+  OCHK(hipMemsetAsync(yp, 0, sizeof(double) * (size_t)total, st));
+  OCHK(hipMemsetAsync(z, 0, sizeof(double) * (size_t)total, st));
+  hipLaunchKernelGGL(k_omega_norm, grid, block, 0, st, total, (const double *)inv, (const double *)y, part);
   OCHK(hipGetLastError());
   double nrm2 = 0.0;
   OCHK(sums(1, &nrm2));
   if (!(nrm2 > 0.0)) return QESMC_ERR_UNSUPPORTED;  // no active cell
-  hipLaunchKernelGGL(k_omega_scale, grid, block, 0, st, total, 1.0 / std::sqrt(nrm2), v);
+  hipLaunchKernelGGL(k_omega_scale, grid, block, 0, st, total, 1.0 / std::sqrt(nrm2), y);
   OCHK(hipGetLastError());
   // Lanczos with one reduction per step: w = M v gives alpha = w.v and
   // |w|^2 together, the next beta from |w - alpha v - beta u|^2 =
@@ -286,11 +345,21 @@ int qesmc_estimate_omega(qesmc_ctx *x, int steps, double *rho, float *omega)
   std::vector<double> hh(2 * (size_t)steps);
   double last = -1.0, r = 0.0;
   for (int it = 0; it < steps; ++it) {
-    hipLaunchKernelGGL(k_omega_apply, grid, block, 0, st, d, cf[0], cf[1], cf[2], cf[3], cf[4], cf[5], x->d_dz, s, v,
-                       w, part);
+    if (masked)
+      hipLaunchKernelGGL(k_omega_apply<true>, grid, block, 0, st, d, cf[0], cf[1], cf[2], cf[3], cf[4], cf[5],
+                         x->d_mask[0], x->d_mask[1], x->d_canon, x->d_dz, (const double *)inv, (const double *)y,
+                         z, part);
+    else
+      hipLaunchKernelGGL(k_omega_apply<false>, grid, block, 0, st, d, cf[0], cf[1], cf[2], cf[3], cf[4], cf[5],
+                         nullptr, nullptr, nullptr, x->d_dz, (const double *)inv, (const double *)y, z, part);
     hipLaunchKernelGGL(k_omega_scalars, dim3(1), block, 0, st, (const double *)part, it, hist);
-    hipLaunchKernelGGL(k_omega_next, grid, block, 0, st, total, it, (const double *)hist, u, v, (const double *)w);
+    hipLaunchKernelGGL(k_omega_next, next_grid, block, 0, st, total, it, (const double *)hist,
+                       (const double *)yp, (const double *)y, z);
     OCHK(hipGetLastError());
+    double *const old = yp;
+    yp = y;
+    y = z;
+    z = old;
     if ((it + 1) % 10 != 0 && it + 1 != steps) continue;
     OCHK(hipMemcpyAsync(hh.data(), hist, sizeof(double) * 2 * (size_t)(it + 1), hipMemcpyDeviceToHost, st));
     OCHK(hipStreamSynchronize(st));
