@@ -209,7 +209,8 @@ def time_to_tol_leg(seeds, fp64_ref, tol_ms=1e-4, chunk=25, cap=6000, omega_step
     reference's omega 1.78 and at the omega qesmc_estimate_omega gives for
     each city (the estimate depends only on the geometry, so one estimate
     serves every wind direction and time step of a city; its cost is listed
-    on its own).
+    on its own), and cold with the cyclic Chebyshev schedule from the
+    estimated rho_J (qesmc_params.chebyshev_rho).
 
     Pass 1 (untimed) runs SOR in chunks of `chunk` iterations and reads the
     velocity back after each, to find the first checkpoint whose L_inf error
@@ -227,7 +228,8 @@ def time_to_tol_leg(seeds, fp64_ref, tol_ms=1e-4, chunk=25, cap=6000, omega_step
         return {"skipped": "trained weights not found: " + WEIGHTS}
     net = W.load(WEIGHTS)
     rows = []
-    runs = (("cold", "ref"), ("warm", "ref"), ("cold", "est"), ("warm", "est"))
+    runs = (("cold", "ref"), ("warm", "ref"), ("cold", "est"), ("warm", "est"), ("cold", "cheb"))
+    suffix = {"ref": "", "est": "_omega_est", "cheb": "_chebyshev"}
     with Q.Context(0) as ctx:
         for seed in seeds:
             city = W.seeded_city(seed)
@@ -238,7 +240,13 @@ def time_to_tol_leg(seeds, fp64_ref, tol_ms=1e-4, chunk=25, cap=6000, omega_step
             row = {"seed": seed, "omega_estimate": {"rho_jacobi": rho, "omega": om_est, "ms": round(est_ms, 3)}}
             for mode, which in runs:
                 omega = REFERENCE_OMEGA if which == "ref" else om_est
-                key = mode if which == "ref" else mode + "_omega_est"
+                key = mode + suffix[which]
+
+                def sched(start):
+                    """the cyclic Chebyshev schedule from the estimated rho_J,
+                    continued across the chunked calls"""
+                    return dict(chebyshev_rho=rho, chebyshev_start=start) if which == "cheb" else {}
+
                 ctx.set_lambda(None)
                 if mode == "warm":
                     net.predict(ctx)
@@ -246,7 +254,7 @@ def time_to_tol_leg(seeds, fp64_ref, tol_ms=1e-4, chunk=25, cap=6000, omega_step
                 if fp64_ref is not None:
                     u64, v64, w64, _ = fp64_ref[seed]
                     while it < cap:
-                        ctx.run(itermax=chunk, tol=-1.0, omega=omega)
+                        ctx.run(itermax=chunk, tol=-1.0, omega=omega, **sched(it))
                         it += chunk
                         u, v, w = ctx.get_velocity()
                         err = max(_linf(u, u64), _linf(v, v64), _linf(w, w64))
@@ -260,10 +268,11 @@ def time_to_tol_leg(seeds, fp64_ref, tol_ms=1e-4, chunk=25, cap=6000, omega_step
                 ctx.set_lambda(None)
                 unet_ms, tflops = net.predict(ctx) if mode == "warm" else (0.0, 0.0)
                 if fp64_ref is not None and it_tol is not None:
-                    r = ctx.run(itermax=it_tol, tol=-1.0, omega=omega)
+                    r = ctx.run(itermax=it_tol, tol=-1.0, omega=omega, **sched(0))
                 else:
-                    r = ctx.run(itermax=20000, tol=1e-4, omega=omega)
-                row[key] = {"omega": omega, "iterations": r["iterations"], "ms": round(r["ms_total"] + unet_ms, 3),
+                    r = ctx.run(itermax=20000, tol=1e-4, omega=omega, **sched(0))
+                row[key] = {"omega": omega if which != "cheb" else "cyclic Chebyshev, rho_J %.8f" % rho,
+                            "iterations": r["iterations"], "ms": round(r["ms_total"] + unet_ms, 3),
                             "sor_ms": round(r["ms_sor"], 3), "err_at_tol": err_tol, "err_at_500": err500}
                 if mode == "warm":
                     row[key].update(unet_ms=round(unet_ms, 3), unet_tflops=round(tflops, 1))
@@ -278,12 +287,13 @@ def time_to_tol_leg(seeds, fp64_ref, tol_ms=1e-4, chunk=25, cap=6000, omega_step
                     # near 1e-4 there)
                     row[key]["stop_rule"] = {}
                     variants = [(n_, e_, 0) for n_, e_ in STOP_RULES]
-                    if which == "est":
+                    if which != "ref":
                         variants += [(n_ + "_refine_every_%d" % REFINE_EVERY, e_, REFINE_EVERY) for n_, e_ in STOP_RULES]
                     for name, ext, ref_every in variants:
                         ctx.set_lambda(None)
                         pred_ms = net.predict(ctx)[0] if mode == "warm" else 0.0
-                        r = ctx.run(itermax=cap, tol=1e-4, stop_extend=ext, omega=omega, refine_every=ref_every)
+                        r = ctx.run(itermax=cap, tol=1e-4, stop_extend=ext, omega=omega, refine_every=ref_every,
+                                    **sched(0))
                         u, v, w = ctx.get_velocity()
                         row[key]["stop_rule"][name] = {
                             "iterations": r["iterations"], "ms": round(r["ms_total"] + pred_ms, 3),
@@ -295,7 +305,7 @@ def time_to_tol_leg(seeds, fp64_ref, tol_ms=1e-4, chunk=25, cap=6000, omega_step
         vals = [r[key][field] for r in rows if r[key].get(field) is not None]
         return float(np.mean(vals)) if vals else None
 
-    keys = [m if w == "ref" else m + "_omega_est" for m, w in runs]
+    keys = [m + suffix[w] for m, w in runs]
     stop = None
     if fp64_ref is not None:
         stop = {}
@@ -316,6 +326,8 @@ def time_to_tol_leg(seeds, fp64_ref, tol_ms=1e-4, chunk=25, cap=6000, omega_step
         stop["speedup_omega_est_refined_vs_reference_production_rule"] = round(
             stop["cold"][prod]["mean_ms"] / min(stop[k][prod_ref]["mean_ms"]
                                                 for k in ("cold_omega_est", "warm_omega_est")), 3)
+        stop["speedup_chebyshev_refined_vs_reference_production_rule"] = round(
+            stop["cold"][prod]["mean_ms"] / stop["cold_chebyshev"][prod_ref]["mean_ms"], 3)
     summary = {}
     for key in keys:
         summary[key] = {"iterations": mean(key, "iterations"), "ms": mean(key, "ms"),
@@ -337,6 +349,7 @@ def time_to_tol_leg(seeds, fp64_ref, tol_ms=1e-4, chunk=25, cap=6000, omega_step
         "speedup_warm_vs_cold": ratio("cold", "warm"),
         "speedup_warm_vs_cold_omega_est": ratio("cold_omega_est", "warm_omega_est"),
         "speedup_omega_est_vs_reference": ratio("cold", "cold_omega_est"),
+        "speedup_chebyshev_vs_omega_est": ratio("cold_omega_est", "cold_chebyshev"),
         "speedup_best_vs_reference_cold": round(summary["cold"]["ms"] / min(summary[k]["ms"] for k in keys), 3),
         "omega_estimate": {"steps_cap": omega_steps,
                            "mean_omega": float(np.mean([r["omega_estimate"]["omega"] for r in rows])),

@@ -114,6 +114,22 @@ typedef struct qesmc_params {
                        In exact arithmetic the iterates are the plain run's.
                        Not in qesmc_run_batch or qesmc_run_static
                        (QESMC_ERR_UNSUPPORTED). */
+  double chebyshev_rho; /* 0: every colour pass at omega.  In (0, 1) (split
+                       path only): the Jacobi spectral radius rho_J (e.g.
+                       from qesmc_estimate_omega), and omega varies by colour
+                       pass as cyclic Chebyshev semi-iteration prescribes
+                       (Golub & Varga 1961): pass h = 0, 1, 2, ... of the
+                       schedule (red, black, red, ...) uses w_0 = 1,
+                       w_1 = 1 / (1 - rho^2 / 2), w_h = 1 / (1 - rho^2 w_h-1 / 4),
+                       which falls to Young's optimum from above; omega is then
+                       unused.  Same fixed point; fewer iterations than a
+                       fixed omega because the early passes are optimal for
+                       the polynomial degree reached.  No counterpart in the
+                       reference.  Not in qesmc_run_batch
+                       (QESMC_ERR_UNSUPPORTED). */
+  int chebyshev_start; /* iterations of the schedule run by earlier calls: a
+                       solve split over several calls passes the running
+                       count so the schedule continues (0: a fresh start). */
 } qesmc_params;
 
 /* qesmc_params.flags: rebuild the stencil encoding from the resident

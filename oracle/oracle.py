@@ -44,6 +44,7 @@ def lib():
     L.oracle_sor_rb_f32.restype = I
     L.oracle_sor_serial_f32.argtypes = L.oracle_sor_rb_f32.argtypes
     L.oracle_sor_serial_f32.restype = I
+    L.oracle_sor_rb_sched_f32.argtypes = [I, I, I] + [f32p] * 6 + [f32p, f32p, f32p, I]
     L.oracle_sor_rb_f64.argtypes = [I, I, I] + [f32p] * 6 + [f64p, f64p, f64p, D, D, I, C.POINTER(D)]
     L.oracle_sor_rb_f64.restype = I
     L.oracle_velocity_f32.argtypes = [I, I, I, Fl, Fl, f32p, f32p, f32p, f32p, i32p,
@@ -130,6 +131,31 @@ def sor_rb_f32(p: Problem, R, lam=None, omega=1.78, tol=1e-9, itermax=500):
     it = lib().oracle_sor_rb_f32(p.nx, p.ny, p.nz, *p.coeffs, np.ascontiguousarray(R, np.float32),
                                  lam, old, omega, tol, itermax, C.byref(me))
     return lam, it, me.value
+
+
+def chebyshev_omegas(rho, passes, start=0):
+    """The factors of colour passes start .. start+passes-1 of cyclic
+    Chebyshev semi-iteration for a 2-cyclic system whose Jacobi matrix has
+    spectral radius rho (Golub & Varga, Numer. Math. 3 (1961) 147-168; Varga,
+    Matrix Iterative Analysis, sec. 5.2): w_0 = 1, w_1 = 1 / (1 - rho^2 / 2),
+    w_h = 1 / (1 - rho^2 w_(h-1) / 4), computed in double and rounded to
+    float32 (the SOR pass's type).  Pass 2t is iteration t's red pass."""
+    r2 = float(rho) * float(rho)
+    out, w = [], 1.0
+    for h in range(start + passes):
+        w = 1.0 if h == 0 else 1.0 / (1.0 - 0.5 * r2) if h == 1 else 1.0 / (1.0 - 0.25 * r2 * w)
+        if h >= start:
+            out.append(w)
+    return np.array(out, np.float32)
+
+
+def sor_rb_sched_f32(p: Problem, R, omegas, lam=None):
+    """len(omegas) // 2 red-black iterations, pass h at omegas[h]."""
+    lam = np.zeros(p.ncell, np.float32) if lam is None else np.array(lam, np.float32, copy=True)
+    om = np.ascontiguousarray(omegas, np.float32)
+    lib().oracle_sor_rb_sched_f32(p.nx, p.ny, p.nz, *p.coeffs, np.ascontiguousarray(R, np.float32), lam, om,
+                                  om.size // 2)
+    return lam
 
 
 def sor_serial_f32(p: Problem, R, lam=None, omega=1.78, tol=1e-9, itermax=500):

@@ -163,6 +163,28 @@ int oracle_sor_rb_f32(int nx, int ny, int nz, const float *e, const float *f, co
   return iter;
 }
 
+/* The red-black loop of oracle_sor_rb_f32 with a factor per colour pass
+ * (omegas[2t] for the red pass of iteration t, omegas[2t+1] for the black
+ * pass) and a fixed count of iterations: the over-relaxation schedules the
+ * library offers beside the reference's fixed factor (cyclic Chebyshev
+ * semi-iteration).  No counterpart in the reference; each pass is the
+ * reference's pass, Solver_CPU_RB.cpp:109-162, at that pass's factor. */
+void oracle_sor_rb_sched_f32(int nx, int ny, int nz, const float *e, const float *f, const float *g,
+                             const float *h, const float *m, const float *n, const float *R,
+                             float *lambda, const float *omegas, int iters)
+{
+  for (int t = 0; t < iters; ++t) {
+#pragma omp parallel
+    {
+      rb_pass_f32(nx, ny, nz, 0, e, f, g, h, m, n, R, lambda, omegas[2 * t]);
+      rb_pass_f32(nx, ny, nz, 1, e, f, g, h, m, n, R, lambda, omegas[2 * t + 1]);
+#pragma omp for schedule(static)
+      for (int j = 0; j < ny - 1; ++j)
+        for (int i = 0; i < nx - 1; ++i) lambda[CELL(i, j, 0)] = lambda[CELL(i, j, 1)];
+    }
+  }
+}
+
 /* Solver_CPU::solve SOR loop, Solver_CPU.cpp:88-140: one in-place
  * lexicographic sweep (k, then j, then i) per iteration, then the mirror and
  * the same max|dlambda| stop test.  Single-threaded as in the reference. */

@@ -23,8 +23,8 @@ hipError_t qesmc_launch_stencil_encode(qesmc_ctx *x);
 hipError_t qesmc_launch_split_coefs(qesmc_ctx *x);
 hipError_t qesmc_launch_pack(qesmc_ctx *x, bool to_split);
 hipError_t qesmc_launch_unpack_selected(qesmc_ctx *x);
-hipError_t qesmc_launch_iteration(qesmc_ctx *x, int path, bool masked, int t, int itermax,
-                                  float omega, float tol);
+hipError_t qesmc_launch_iteration(qesmc_ctx *x, int path, bool masked, int t, int itermax, float omega_red,
+                                  float omega_black, float tol);
 hipError_t qesmc_launch_finish(qesmc_ctx *x, bool mirror, int alpha1, int alpha2);
 hipError_t qesmc_launch_div_linf(qesmc_ctx *x, int physical);
 hipError_t qesmc_launch_set_u32(qesmc_ctx *x, unsigned *p, unsigned v);
@@ -231,6 +231,8 @@ void qesmc_default_params(qesmc_params *p)
   p->slab_planes = 0;
   p->stop_extend = 0.f;
   p->refine_every = 0;
+  p->chebyshev_rho = 0.0;
+  p->chebyshev_start = 0;
 }
 
 int qesmc_create(qesmc_ctx **out, int device)
@@ -371,8 +373,9 @@ int run_prepare(qesmc_ctx *x, const qesmc_params *pp, RunState &st)
   if (!(p.stop_extend >= 0.f && p.stop_extend <= 1000.f)) return QESMC_ERR_ARG;  // also refuses NaN
   if (p.tile_v != 0 && p.tile_v != 1 && p.tile_v != 2 && p.tile_v != 4) return QESMC_ERR_ARG;
   if (p.refine_every < 0) return QESMC_ERR_ARG;
+  if (!(p.chebyshev_rho >= 0.0 && p.chebyshev_rho < 1.0) || p.chebyshev_start < 0) return QESMC_ERR_ARG;
   st.path = (p.path == QESMC_PATH_PLAIN || p.path == QESMC_PATH_FUSED) ? p.path : QESMC_PATH_SPLIT;
-  if (p.refine_every > 0 && st.path != QESMC_PATH_SPLIT) return QESMC_ERR_UNSUPPORTED;
+  if ((p.refine_every > 0 || p.chebyshev_rho > 0.0) && st.path != QESMC_PATH_SPLIT) return QESMC_ERR_UNSUPPORTED;
   HIPCHK(hipSetDevice(x->device));
   if (p.flags & QESMC_RUN_REENCODE) {
     int rc = encode_stencil(x);
@@ -549,8 +552,25 @@ int run_loop(qesmc_ctx *x, const RunState &st, int *launched_out)
     // result never depends on when the host notices.
     int chunk = 0;
     bool stop = false;
+    // the cyclic Chebyshev schedule, advanced past the passes of earlier calls
+    const double rho2 = p.chebyshev_rho * p.chebyshev_rho;
+    long long pass = 0;
+    double w_prev = 1.0;
+    auto next_omega = [&]() -> float {
+      const double w = pass == 0 ? 1.0 : pass == 1 ? 1.0 / (1.0 - 0.5 * rho2) : 1.0 / (1.0 - 0.25 * rho2 * w_prev);
+      ++pass;
+      w_prev = w;
+      return (float)w;
+    };
+    if (p.chebyshev_rho > 0.0)
+      for (long long h = 0; h < 2LL * p.chebyshev_start; ++h) (void)next_omega();
     for (int t = 0; t < p.itermax && !stop; ++t) {
-      hipError_t e = qesmc_launch_iteration(x, st.path, st.masked, t, p.itermax, p.omega, p.tol);
+      float w_red = p.omega, w_black = p.omega;
+      if (p.chebyshev_rho > 0.0) {
+        w_red = next_omega();
+        w_black = next_omega();
+      }
+      hipError_t e = qesmc_launch_iteration(x, st.path, st.masked, t, p.itermax, w_red, w_black, p.tol);
       if (e != hipSuccess) {
         rc = map_hip(e);
         break;

@@ -239,7 +239,7 @@ int qesmc_run_batch(qesmc_ctx **ctxs, int n, const qesmc_params *params, qesmc_r
   }
   if (params->path == QESMC_PATH_PLAIN || params->path == QESMC_PATH_FUSED) return QESMC_ERR_UNSUPPORTED;
   // the batched loop has no refinement refresh
-  if (params->refine_every != 0) return QESMC_ERR_UNSUPPORTED;
+  if (params->refine_every != 0 || params->chebyshev_rho != 0.0) return QESMC_ERR_UNSUPPORTED;
   // One tiling for the whole batch (half-cells per lane 1 or 2, planes per
   // wave tile) unless the caller sets it: the most work per wave tile, at
   // most 4 planes x 2 half-cells, that leaves >= 6 wave tiles per CU over all

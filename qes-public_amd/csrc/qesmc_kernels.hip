@@ -465,8 +465,8 @@ static const unsigned kPlaneBlocks = 8;  // work-groups per plane of the plane r
 using namespace qesmc;
 
 hipError_t qesmc_launch_fused_iteration(qesmc_ctx *x, int t, int itermax, float omega, float tol);
-hipError_t qesmc_launch_split_iteration(qesmc_ctx *x, bool masked, int t, int itermax, float omega,
-                                        float tol);
+hipError_t qesmc_launch_split_iteration(qesmc_ctx *x, bool masked, int t, int itermax, float omega_red,
+                                        float omega_black, float tol);
 
 // Entry points used by the API translation unit.
 hipError_t qesmc_launch_divergence(qesmc_ctx *x, bool split, int alpha1)
@@ -528,9 +528,12 @@ hipError_t qesmc_launch_unpack_selected(qesmc_ctx *x)
   return hipGetLastError();
 }
 
+// omega_black applies to the split path's black pass; the other paths run
+// both colours at omega_red (the run refuses a schedule there).
 hipError_t qesmc_launch_iteration(qesmc_ctx *x, int path, bool masked, int t, int itermax,
-                                  float omega, float tol)
+                                  float omega_red, float omega_black, float tol)
 {
+  const float omega = omega_red;
   const Dims &d = x->d;
   if (d.NZ - 2 < 1 || d.NY - 2 < 1) return hipSuccess;
   const float omo = 1.0f - omega;
@@ -544,7 +547,7 @@ hipError_t qesmc_launch_iteration(qesmc_ctx *x, int path, bool masked, int t, in
   } else if (path == QESMC_PATH_FUSED) {
     return qesmc_launch_fused_iteration(x, t, itermax, omega, tol);
   } else {
-    return qesmc_launch_split_iteration(x, masked, t, itermax, omega, tol);
+    return qesmc_launch_split_iteration(x, masked, t, itermax, omega_red, omega_black, tol);
   }
   return hipGetLastError();
 }

@@ -188,8 +188,8 @@ hipError_t launch_split_pass(qesmc_ctx *x, bool masked, int colour, int k_lo, in
 // the bytes come from: the black launch re-reads the lambda planes that the
 // red launch just read and wrote, from the Infinity Cache instead of HBM,
 // when a slab's footprint fits in it.
-hipError_t qesmc_launch_split_iteration(qesmc_ctx *x, bool masked, int t, int itermax, float omega,
-                                        float tol)
+hipError_t qesmc_launch_split_iteration(qesmc_ctx *x, bool masked, int t, int itermax, float omega_red,
+                                        float omega_black, float tol)
 {
   const Dims &d = x->d;
   if (d.NZ - 2 < 1 || d.NY - 2 < 1) return hipSuccess;
@@ -201,17 +201,17 @@ hipError_t qesmc_launch_split_iteration(qesmc_ctx *x, bool masked, int t, int it
   const int lo = 1, hi = d.NZ - 1;  // updated planes [1, NZ-2]
   const int S = x->slab_planes;
   if (S <= 0 || S >= hi - lo) {
-    hipError_t e = launch_split_pass(x, masked, 0, lo, hi, t, itermax, omega, tol, reverse);
+    hipError_t e = launch_split_pass(x, masked, 0, lo, hi, t, itermax, omega_red, tol, reverse);
     if (e != hipSuccess) return e;
-    return launch_split_pass(x, masked, 1, lo, hi, t, itermax, omega, tol, reverse);
+    return launch_split_pass(x, masked, 1, lo, hi, t, itermax, omega_black, tol, reverse);
   }
   int black_lo = lo;
   for (int a = lo; a < hi; a += S) {
     const int b = a + S < hi ? a + S : hi;
-    hipError_t e = launch_split_pass(x, masked, 0, a, b, t, itermax, omega, tol, reverse);
+    hipError_t e = launch_split_pass(x, masked, 0, a, b, t, itermax, omega_red, tol, reverse);
     if (e != hipSuccess) return e;
     const int black_hi = b == hi ? hi : b - 1;
-    e = launch_split_pass(x, masked, 1, black_lo, black_hi, t, itermax, omega, tol, reverse);
+    e = launch_split_pass(x, masked, 1, black_lo, black_hi, t, itermax, omega_black, tol, reverse);
     if (e != hipSuccess) return e;
     black_lo = black_hi;
   }

@@ -74,7 +74,8 @@ class Params(C.Structure):
                 ("alpha1", C.c_int), ("alpha2", C.c_int), ("path", C.c_int),
                 ("check_every", C.c_int), ("flags", C.c_int), ("tile_k", C.c_int),
                 ("tile_v", C.c_int), ("wg_per_cu", C.c_int), ("slab_planes", C.c_int),
-                ("stop_extend", C.c_float), ("refine_every", C.c_int)]
+                ("stop_extend", C.c_float), ("refine_every", C.c_int),
+                ("chebyshev_rho", C.c_double), ("chebyshev_start", C.c_int)]
 
 
 class Result(C.Structure):

@@ -42,7 +42,8 @@ def test_struct_layouts_match_c_compiler(qesmc, tmp_path):
         "qesmc_coeffs": (qesmc.Coeffs, ["e", "n", "icellflag"]),
         "qesmc_fields": (qesmc.Fields, ["u0", "w"]),
         "qesmc_params": (qesmc.Params, ["omega", "tol", "itermax", "path", "flags", "tile_k", "wg_per_cu",
-                                         "slab_planes", "stop_extend", "refine_every"]),
+                                         "slab_planes", "stop_extend", "refine_every", "chebyshev_rho",
+                                         "chebyshev_start"]),
         "qesmc_result": (qesmc.Result, ["iterations", "status", "final_max_dlambda", "glups", "tile_k", "reserved"]),
         "qesmc_building": (qesmc.Building, ["x_start", "rotation_deg"]),
         "qesmc_inflow": (qesmc.Inflow, ["kind", "wind_dir_deg"]),

@@ -316,6 +316,45 @@ def test_loop_entry_conditions(oracle, itermax, tol, expect_iters):
         assert np.array_equal(lam, lam0)
 
 
+def test_chebyshev_factors_closed_form(oracle):
+    """The cyclic Chebyshev factors from their recurrence against the
+    Chebyshev-polynomial closed form w_h = 2 C_h(1/rho) / (rho C_(h+1)(1/rho))
+    (h >= 1; C_n(x) = cosh(n acosh x)), w_0 = 1, and the limit, Young's
+    optimum 2 / (1 + sqrt(1 - rho^2)); a running start continues the same
+    sequence."""
+    O = oracle
+    for rho in (0.5, 0.9, 0.99):
+        w = O.chebyshev_omegas(rho, 60).astype(np.float64)
+        a = np.arccosh(1.0 / rho)
+        h = np.arange(1, 60)
+        closed = 2.0 * np.cosh(h * a) / (rho * np.cosh((h + 1) * a))
+        assert w[0] == 1.0
+        np.testing.assert_allclose(w[1:], closed, rtol=2e-7)
+        assert np.all(np.diff(w[1:]) <= 0)  # falls to the optimum from above
+        lim = O.chebyshev_omegas(rho, 1, start=4000)[0]
+        assert lim == np.float32(2.0 / (1.0 + np.sqrt(1.0 - rho * rho)))
+        assert np.array_equal(O.chebyshev_omegas(rho, 10, start=7), O.chebyshev_omegas(rho, 17)[7:])
+
+
+def test_scheduled_loop_pass_factors(oracle):
+    """A constant schedule is the fixed-omega loop bit for bit; and pass 2t is
+    the red pass: a factor of 0 leaves that colour's cells as they were."""
+    O = oracle
+    p = small_city(O, 20, 16, 9)
+    R = O.divergence_f32(p)
+    ref = O.sor_rb_f32(p, R, None, omega=1.6, itermax=7, tol=-1.0)[0]
+    assert np.array_equal(O.sor_rb_sched_f32(p, R, np.full(14, 1.6, np.float32)), ref)
+    NX, NY, NZ = p.nx - 1, p.ny - 1, p.nz - 1
+    k, j, i = np.meshgrid(np.arange(NZ), np.arange(NY), np.arange(NX), indexing="ij")
+    inner = (i >= 1) & (i <= NX - 2) & (j >= 1) & (j <= NY - 2) & (k >= 1) & (k <= NZ - 2)
+    red = (inner & ((i + j + k) % 2 == 0)).ravel()
+    black = (inner & ((i + j + k) % 2 == 1)).ravel()
+    a = O.sor_rb_sched_f32(p, R, np.array([0.0, 1.0], np.float32))
+    assert np.all(a[red] == 0) and np.count_nonzero(a[black]) > 0
+    b = O.sor_rb_sched_f32(p, R, np.array([1.0, 0.0], np.float32))
+    assert np.all(b[black] == 0) and np.count_nonzero(b[red]) > 0
+
+
 # ---------------------------------------------------------------------------
 # inputs: grid, flags, walls, coefficients, profiles (closed forms)
 # ---------------------------------------------------------------------------
