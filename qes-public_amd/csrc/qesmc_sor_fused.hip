@@ -25,11 +25,17 @@
 // that the re-reads mostly hit in L2.  Per iteration a cell costs its own
 // lambda read and write, R and the mask: 13 bytes, against 2 x 17 per cell
 // pair for two split colour passes, plus the halo re-reads (K+4 black planes
-// loaded per K relaxed, W+2 lanes per W, L+2 red rows per L).  As built it
-// is bound by instruction issue, not by bytes: the halo relaxations, lane
-// shuffles and the register rotation of the march about double the split
-// kernel's arithmetic per update (loading the next step during the current
-// one changed nothing), so the split kernel stays the default.
+// loaded per K relaxed, W+2 lanes per W, L+2 red rows per L).  The factor
+// omega / (e+f+g+h+m+n) of a cell depends only on its layer and its 6-bit
+// mask, so each wave tabulates it for its planes in LDS (with the same
+// expression, hence the same value) instead of dividing per cell; with the
+// coefficients selected by bit masks this took a quarter of the march's
+// vector instructions away.  On grids beyond the Infinity Cache the march
+// is then bound by its HBM traffic: the halos, fetched by whole cache lines
+// (a 58-lane row of 232 B is not line aligned) and not reliably shared
+// through L2 by the neighbouring tiles, bring it to about 1.5x its 13 bytes
+// per update, close to what two split passes move, so the split kernel
+// stays the default.
 #include "qesmc_sor_tile.h"
 
 namespace qesmc {
@@ -46,17 +52,29 @@ __device__ inline float wave_max(float v)
   return v;
 }
 
-__device__ __forceinline__ float relax_masked(unsigned m, const float *cv, float xp, float xm, float yp, float ym,
-                                              float zp, float zm, float r, float o, float omega, float one_m_omega)
+// Wall value a of a layer (its bit pattern cb) where mask bit a is set, +0
+// where it is not: the bit sign-extended to a word and ANDed with the value.
+template <int A>
+__device__ __forceinline__ float wall_coef(unsigned m, unsigned cb)
 {
-  return relax((m & 1u) ? cv[0] : 0.f, (m & 2u) ? cv[1] : 0.f, (m & 4u) ? cv[2] : 0.f, (m & 8u) ? cv[3] : 0.f,
-               (m & 16u) ? cv[4] : 0.f, (m & 32u) ? cv[5] : 0.f, xp, xm, yp, ym, zp, zm, r, o, omega, one_m_omega);
+  return __uint_as_float(cb & (unsigned)((int)(m << (31 - A)) >> 31));
+}
+
+// relax() of qesmc_sor_tile.h with its factor omega / (e+f+g+h+m+n) read
+// from a table (wd, computed by the same expression) instead of divided per
+// cell: the same operations on the same values, so the same result.
+__device__ __forceinline__ float relax_tab(unsigned m, const unsigned *cb, float wd, float xp, float xm, float yp,
+                                           float ym, float zp, float zm, float r, float o, float one_m_omega)
+{
+  const float ec = wall_coef<0>(m, cb[0]), fc = wall_coef<1>(m, cb[1]), gc = wall_coef<2>(m, cb[2]),
+              hc = wall_coef<3>(m, cb[3]), mc = wall_coef<4>(m, cb[4]), nc = wall_coef<5>(m, cb[5]);
+  return wd * (ec * xp + fc * xm + gc * yp + hc * ym + mc * zp + nc * zm - r) + one_m_omega * o;
 }
 
 }  // namespace
 
 template <int K>
-__global__ __launch_bounds__(256) void k_sor_march(Dims d, int W, int qt, int L, int nkc, long long ntiles,
+__global__ __launch_bounds__(256, 3) void k_sor_march(Dims d, int W, int qt, int L, int nkc, long long ntiles,
                                                    const float *__restrict__ a0, const float *__restrict__ a1,
                                                    float *__restrict__ b0, float *__restrict__ b1,
                                                    const float *__restrict__ R0, const float *__restrict__ R1,
@@ -69,6 +87,10 @@ __global__ __launch_bounds__(256) void k_sor_march(Dims d, int W, int qt, int L,
   constexpr int NB = K + 4;  // black-old plane slots: planes k0-2 .. k0+K+1
   constexpr int NR = K + 2;  // red plane slots: planes k0-1 .. k0+K
   __shared__ float wmax[kWaves];
+  // per wave and relaxed plane k0-1+p: omega / (sum of the wall values a
+  // cell's mask selects), for each of the 64 masks, and the six wall values
+  __shared__ float wd_tab[kWaves][NR][64];
+  __shared__ unsigned cv_tab[kWaves][NR][8];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (t >= itermax) return;
@@ -84,6 +106,27 @@ __global__ __launch_bounds__(256) void k_sor_march(Dims d, int W, int qt, int L,
   const long long blk = (long long)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
   if (blk == 0 && threadIdx.x == 0) *cur = (unsigned)((t + 1) & 1);
   const long long tile = blk * kWaves + wave;
+  {
+    // this wave's tables (planes clamped into the domain; a tile past the
+    // end fills harmless values it never reads)
+    const int k0t = 1 + (int)(tile % nkc) * K;
+    for (int e = lane; e < NR * 64; e += 64) {
+      const int p = e >> 6, mm = e & 63;
+      int k = k0t - 1 + p;
+      k = k < 0 ? 0 : k > d.NZ - 1 ? d.NZ - 1 : k;
+      float v[6];
+#pragma unroll
+      for (int a = 0; a < 6; ++a) v[a] = ((mm >> a) & 1) ? __uint_as_float(canon8[k * 8 + a]) : 0.f;
+      wd_tab[wave][p][mm] = omega / (v[0] + v[1] + v[2] + v[3] + v[4] + v[5]);
+    }
+    for (int e = lane; e < NR * 8; e += 64) {
+      const int p = e >> 3;
+      int k = k0t - 1 + p;
+      k = k < 0 ? 0 : k > d.NZ - 1 ? d.NZ - 1 : k;
+      cv_tab[wave][p][e & 7] = canon8[k * 8 + (e & 7)];
+    }
+  }
+  __syncthreads();
 
   float dl = 0.f;
   bool bad = false;
@@ -96,7 +139,9 @@ __global__ __launch_bounds__(256) void k_sor_march(Dims d, int W, int qt, int L,
     const int k1 = k0 + K < d.NZ - 1 ? k0 + K : d.NZ - 1;  // owned planes [k0, k1)
     const int j0 = 1 + tj * L;
     const int j1 = j0 + L < d.NY - 1 ? j0 + L : d.NY - 1;  // owned rows [j0, j1)
-    const int q = tq * W - 1 + lane;
+    // lanes past the right halo (W + 2 <= 64) repeat its address: they load
+    // nothing the wave does not already fetch, and own nothing
+    const int q = tq * W - 1 + (lane < W + 2 ? lane : W + 1);
     const bool qv = q >= 0 && q < d.HP;
     const unsigned qc = qv ? (unsigned)q : 0u;
     const bool own = qv && lane >= 1 && lane <= W;
@@ -163,18 +208,16 @@ __global__ __launch_bounds__(256) void k_sor_march(Dims d, int W, int qt, int L,
       // of this plane, planes k-1 and k+1 of row j-1 from the last step)
       const bool rowu = j >= 1 && j <= d.NY - 2;
       const bool rown = j >= j0 && j < j1;
-      // the wall values of a plane: scalar loads every step (cached),
-      // rather than all planes' held in scalar registers for the whole march
-      const unsigned *cg = canon8;
-      asm volatile("" : "+s"(cg));
-      const __attribute__((address_space(4))) unsigned *cn = (const __attribute__((address_space(4))) unsigned *)cg;
 #pragma unroll
       for (int p = 0; p < NR; ++p) {
         const int k = k0 - 1 + p;
-        const int kc = k < d.NZ ? k : d.NZ - 1;
-        float cv[6];
+        // the plane's wall values (LDS broadcast reads), read here rather
+        // than hoisted for all planes at once, which would hold 6 more
+        // VGPRs per plane through the step
+        asm volatile("" ::: "memory");
+        unsigned cv[6];
 #pragma unroll
-        for (int a = 0; a < 6; ++a) cv[a] = __uint_as_float(cn[kc * 8 + a]);
+        for (int a = 0; a < 6; ++a) cv[a] = cv_tab[wave][p][a];
         {
           const int s = (j + k) & 1;
           const bool upd = (rowu && k >= 1 && k <= d.NZ - 2) && (s ? col1 : col0);
@@ -183,8 +226,8 @@ __global__ __launch_bounds__(256) void k_sor_march(Dims d, int W, int qt, int L,
           const float cup = __shfl_down(c, 1, 64);  // half-cell q+1
           const float xm = s ? c : cdn;
           const float xp = s ? cup : c;
-          const float nv = relax_masked(rmk[p], cv, xp, xm, bp[p + 1], bm[p + 1], bc[p + 2], bc[p], rr[p], ro[p],
-                                        omega, one_m_omega);
+          const float nv = relax_tab(rmk[p], cv, wd_tab[wave][p][rmk[p] & 63u], xp, xm, bp[p + 1], bm[p + 1],
+                                     bc[p + 2], bc[p], rr[p], ro[p], one_m_omega);
           rn[p] = upd ? nv : ro[p];
           if (rown && k >= k0 && k < k1) {  // wave-uniform; k >= k0: never the halo plane
             const bool cnt = own && upd;
@@ -206,8 +249,8 @@ __global__ __launch_bounds__(256) void k_sor_march(Dims d, int W, int qt, int L,
           const float xm = s ? c : cdn;
           const float xp = s ? cup : c;
           const float o = bm[p + 1];
-          const float nv = relax_masked(bmk[p - 1], cv, xp, xm, rn[p], rm2[p], rm1[p + 1], rm1[p - 1], br[p - 1], o,
-                                        omega, one_m_omega);
+          const float nv = relax_tab(bmk[p - 1], cv, wd_tab[wave][p][bmk[p - 1] & 63u], xp, xm, rn[p], rm2[p],
+                                     rm1[p + 1], rm1[p - 1], br[p - 1], o, one_m_omega);
           const bool cnt = own && upd;
           const float dd = cnt ? fabsf(nv - o) : 0.f;
           dl = (dd > dl) ? dd : dl;
