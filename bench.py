@@ -540,7 +540,7 @@ def refined_search(work, ref_dir, tol_ms, basis_tol, omega, chunk, cap):
 
 
 def c5_tol_leg(tol_ms=1e-4, dirs=8, cap=20000, cpu_ms_per_it=None, omega_steps=200, chunk=125,
-               basis_tol=2.5e-5):
+               basis_tol=6.5e-5):
     """Time to tolerance of the `dirs`-direction ensemble on the 1000 x 1000 x
     150 city, against the FP64 fixture (sampled: two x-z and two y-z slices of
     the lowest 100 layers and 131072 random faces; the fixture's FP64 fields
@@ -558,7 +558,10 @@ def c5_tol_leg(tol_ms=1e-4, dirs=8, cap=20000, cpu_ms_per_it=None, omega_steps=2
                   omega = 2, sets no floor), from lambda = 0 and from the
                   U-Net's lambda0: T_tol, and the iterations to `basis_tol`,
                   to which the faster start solves the basis so that the
-                  combinations below stay within tol_ms;
+                  combinations below stay within tol_ms: at every sampled
+                  face a combination's error is at most |cos t| e_270 +
+                  |sin t| e_180 <= sqrt(2) basis_tol (9.2e-5 m/s for the
+                  default 6.5e-5), and each is measured;
       superposed  every other direction from cos(t) lambda_270 + sin(t)
                   lambda_180, refined in chunks of 25 iterations if needed.
     Iteration counts come from chained refinements with checks in between
