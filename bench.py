@@ -204,7 +204,7 @@ REFINE_EVERY = 100
 STOP_RULES = (("reference_tol_1e-4", 0.0), ("production_tol_1e-4_extend_0.3", 0.3))
 
 
-def time_to_tol_leg(seeds, fp64_ref, tol_ms=1e-4, chunk=25, cap=6000, omega_steps=200):
+def time_to_tol_leg(seeds, fp64_ref, tol_ms=1e-4, chunk=25, cap=6000, omega_steps=100):
     """Cold versus U-Net warm start on held-out 128 x 128 x 64 cities, at the
     reference's omega 1.78 and at the omega qesmc_estimate_omega gives for
     each city (the estimate depends only on the geometry, so one estimate
@@ -539,7 +539,7 @@ def refined_search(work, ref_dir, tol_ms, basis_tol, omega, chunk, cap):
     return it_tol, it_basis, hist
 
 
-def c5_tol_leg(tol_ms=1e-4, dirs=8, cap=20000, cpu_ms_per_it=None, omega_steps=200, chunk=125,
+def c5_tol_leg(tol_ms=1e-4, dirs=8, cap=20000, cpu_ms_per_it=None, omega_steps=100, chunk=125,
                basis_tol=6.5e-5):
     """Time to tolerance of the `dirs`-direction ensemble on the 1000 x 1000 x
     150 city, against the FP64 fixture (sampled: two x-z and two y-z slices of

@@ -121,7 +121,7 @@ public:
       rc = qesmc_set_geometry(ctx_, &grid, &coeffs);
       if (rc == QESMC_OK) rc = qesmc_set_inflow(ctx_, WGD->u0.data(), WGD->v0.data(), WGD->w0.data());
       if (rc == QESMC_OK) rc = qesmc_set_lambda(ctx_, lambda.data());
-      if (rc == QESMC_OK && !(omega_est_ > 0.f)) rc = qesmc_estimate_omega(ctx_, 200, nullptr, &omega_est_);
+      if (rc == QESMC_OK && !(omega_est_ > 0.f)) rc = qesmc_estimate_omega(ctx_, 100, nullptr, &omega_est_);
       params.omega = omega_est_;
       params.tol = acc_tol_;
       params.stop_extend = acc_extend_;

@@ -6,7 +6,7 @@ large-domain ensemble: the fp32 device field within 1e-4 m/s (L_inf over the
 sampled u, v, w) of the FP64 solution, from a cold start at the iteration
 counts the benchmark reports: at the reference's omega 1.78 with the plain
 fp32 iteration, and at the omega qesmc_estimate_omega gives for this grid
-(1.9920) with the residual re-formed in FP64 every 125 iterations
+(1.9919 after 100 Lanczos steps) with the residual re-formed in FP64 every 125 iterations
 (refine_every; without it the fp32 rounding noise, which SOR amplifies as
 omega nears 2, holds the sampled error at 1.0-1.3e-4).  The two basis
 directions are solved to 6.5e-5, and other directions come from their
@@ -62,8 +62,8 @@ def test_city1000_within_1e4_of_fp64(qesmc):
         err = b.probe_error(work, b.c5_reference(ref, wd))
         print(f"{wd} deg cold, omega {om}, {its} iterations: sampled error {err:.3e} m/s")
         assert err <= 1e-4, (wd, err)
-        _, om = work.estimate_omega(200)
-        assert abs(om - 1.99198) <= 1e-4, om
+        _, om = work.estimate_omega(100)
+        assert abs(om - 1.99189) <= 1e-4, om
         wd, its = COLD_ESTIMATED
         work.set_city_inflow((1, u_ref, 10.0, 0.5, wd))
         work.set_lambda(None)
