@@ -143,7 +143,11 @@ def chebyshev_omegas(rho, passes, start=0):
     r2 = float(rho) * float(rho)
     out, w = [], 1.0
     for h in range(start + passes):
+        prev = w
         w = 1.0 if h == 0 else 1.0 / (1.0 - 0.5 * r2) if h == 1 else 1.0 / (1.0 - 0.25 * r2 * w)
+        if h >= 2 and w == prev:  # the fixed point: every later factor is w
+            out.extend([w] * (start + passes - max(h, start)))
+            break
         if h >= start:
             out.append(w)
     return np.array(out, np.float32)

@@ -552,18 +552,26 @@ int run_loop(qesmc_ctx *x, const RunState &st, int *launched_out)
     // result never depends on when the host notices.
     int chunk = 0;
     bool stop = false;
-    // the cyclic Chebyshev schedule, advanced past the passes of earlier calls
+    // the cyclic Chebyshev schedule, advanced past the passes of earlier calls;
+    // once a factor repeats the previous one the recurrence has reached its
+    // fixed point (Young's optimum) and every later factor is that value, so
+    // the advance stops there
     const double rho2 = p.chebyshev_rho * p.chebyshev_rho;
     long long pass = 0;
     double w_prev = 1.0;
+    bool fixed = false;
     auto next_omega = [&]() -> float {
-      const double w = pass == 0 ? 1.0 : pass == 1 ? 1.0 / (1.0 - 0.5 * rho2) : 1.0 / (1.0 - 0.25 * rho2 * w_prev);
-      ++pass;
-      w_prev = w;
-      return (float)w;
+      if (!fixed) {
+        const double w =
+            pass == 0 ? 1.0 : pass == 1 ? 1.0 / (1.0 - 0.5 * rho2) : 1.0 / (1.0 - 0.25 * rho2 * w_prev);
+        fixed = pass >= 2 && w == w_prev;
+        ++pass;
+        w_prev = w;
+      }
+      return (float)w_prev;
     };
     if (p.chebyshev_rho > 0.0)
-      for (long long h = 0; h < 2LL * p.chebyshev_start; ++h) (void)next_omega();
+      for (long long h = 0; h < 2LL * p.chebyshev_start && !fixed; ++h) (void)next_omega();
     for (int t = 0; t < p.itermax && !stop; ++t) {
       float w_red = p.omega, w_black = p.omega;
       if (p.chebyshev_rho > 0.0) {

@@ -7,9 +7,12 @@
   (2) a solve split over calls that pass the running count
       (chebyshev_start) is the one-call solve, bit for bit, and a run with
       refine_every = K is K-iteration refinements with running starts;
-  (3) qesmc_run_static gives qesmc_run's result;
+  (3) qesmc_run_static gives qesmc_run's result, and a far running start
+      runs at the recurrence's limit, Young's optimum;
   (4) out-of-range factors and the paths without a per-colour factor are
       refused."""
+import time
+
 import numpy as np
 import pytest
 
@@ -77,6 +80,27 @@ def test_running_start_continues_the_schedule(qesmc, oracle):
     assert np.array_equal(chained[0], refined[0])
     for a, b in zip(chained[1], refined[1]):
         assert np.array_equal(a, b)
+
+
+def test_far_start_runs_at_the_limit(qesmc, oracle):
+    """Past the recurrence's fixed point every pass runs at Young's optimum
+    2 / (1 + sqrt(1 - rho^2)): a far running start equals the fixed factor,
+    and the host does not advance the schedule 2^31 passes to get there."""
+    p = _city(oracle, qesmc, 35, 36, 30, 11)
+    rho = 0.99
+    w = float(np.float32(2.0 / (1.0 + np.sqrt(1.0 - rho * rho))))
+    with qesmc.Context(0) as ctx:
+        _load(ctx, p)
+        ctx.set_lambda(None)
+        ctx.run(itermax=20, tol=-1.0, omega=w)
+        ref = ctx.get_lambda()
+        ctx.set_lambda(None)
+        t0 = time.perf_counter()
+        ctx.run(itermax=20, tol=-1.0, chebyshev_rho=rho, chebyshev_start=2 ** 31 - 1)
+        wall = time.perf_counter() - t0
+        got = ctx.get_lambda()
+    assert wall < 5.0
+    assert np.array_equal(ref, got)
 
 
 def test_static_entry_runs_the_schedule(qesmc, oracle):

@@ -334,6 +334,10 @@ def test_chebyshev_factors_closed_form(oracle):
         lim = O.chebyshev_omegas(rho, 1, start=4000)[0]
         assert lim == np.float32(2.0 / (1.0 + np.sqrt(1.0 - rho * rho)))
         assert np.array_equal(O.chebyshev_omegas(rho, 10, start=7), O.chebyshev_omegas(rho, 17)[7:])
+        # past the recurrence's fixed point every factor is the limit (and a
+        # far start returns at once)
+        assert np.array_equal(O.chebyshev_omegas(rho, 5, start=4000), O.chebyshev_omegas(rho, 4005)[4000:])
+        assert np.all(O.chebyshev_omegas(rho, 3, start=2 ** 40) == lim)
 
 
 def test_scheduled_loop_pass_factors(oracle):
