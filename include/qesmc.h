@@ -68,7 +68,12 @@ typedef struct qesmc_fields {
 } qesmc_fields;
 
 /* Kernel path selection (qesmc_params.path). */
-#define QESMC_PATH_AUTO 0     /* fastest available                              */
+#define QESMC_PATH_AUTO 0     /* fastest available: FUSED where the colour
+                                 arrays exceed the 256 MB Infinity Cache and
+                                 the run allows it (wall mask, no refinement
+                                 or schedule, below 2^31 half-cells, its
+                                 buffers already allocated in a
+                                 qesmc_run_static call), SPLIT otherwise        */
 #define QESMC_PATH_PLAIN 1    /* in-place red/black launches on the x-fastest
                                  layout, full fp32 coefficients: the classic
                                  GPU red-black SOR (baseline)                   */

@@ -103,6 +103,7 @@ struct qesmc_ctx {
   bool have_geom = false, have_inflow = false;
   bool have_velocity = false;  // d_u, d_v, d_w hold a corrected field of the resident geometry
   bool geom_checked = false;   // stencil encoding built for the current geometry
+  bool no_alloc_run = false;   // inside qesmc_run_static: the run may not allocate
   bool compressible = false;
 
   // natural (x-fastest) device arrays

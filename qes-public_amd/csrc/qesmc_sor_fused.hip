@@ -20,22 +20,24 @@
 // planes, whose six red neighbours (rows j-2..j, planes k-1..k+1, and the
 // x-neighbours in the adjacent lanes) are all in registers.  Each black row
 // of A is loaded once, as row j+1, and kept while the march passes it.
-// There is no LDS and no barrier: the halos are recomputed, and the tiles
-// that share them (neighbouring k-chunks first) are dispatched together so
-// that the re-reads mostly hit in L2.  Per iteration a cell costs its own
-// lambda read and write, R and the mask: 13 bytes, against 2 x 17 per cell
-// pair for two split colour passes, plus the halo re-reads (K+4 black planes
-// loaded per K relaxed, W+2 lanes per W, L+2 red rows per L).  The factor
+// The halos are recomputed rather than exchanged, and the tiles that share
+// them are dispatched together so that the re-reads mostly hit in L2: the
+// four waves of a work-group are x-neighbours (a 58-lane row segment is not
+// line aligned, so neighbours share cache lines), the k-neighbours a few
+// work-groups on.  Per iteration a cell costs its own lambda read and
+// write, R and the mask: 13 bytes per update, against 17 in a split colour
+// pass (which also reads the other colour's lambda), plus the halo re-reads
+// (K+4 black planes loaded per K relaxed, W+2 lanes per W, L+2 red rows per
+// L).  The factor
 // omega / (e+f+g+h+m+n) of a cell depends only on its layer and its 6-bit
 // mask, so each wave tabulates it for its planes in LDS (with the same
 // expression, hence the same value) instead of dividing per cell; with the
 // coefficients selected by bit masks this took a quarter of the march's
 // vector instructions away.  On grids beyond the Infinity Cache the march
-// is then bound by its HBM traffic: the halos, fetched by whole cache lines
-// (a 58-lane row of 232 B is not line aligned) and not reliably shared
-// through L2 by the neighbouring tiles, bring it to about 1.5x its 13 bytes
-// per update, close to what two split passes move, so the split kernel
-// stays the default.
+// is bound by its HBM traffic, about 1.25 x 13 bytes per update with the
+// halos, against 1.17 x 17 for the split passes:
+// QESMC_PATH_AUTO picks it there and the split kernel where the arrays stay
+// in the cache, whose bandwidth the split passes use better.
 #include "qesmc_sor_tile.h"
 
 namespace qesmc {
@@ -100,8 +102,9 @@ __global__ __launch_bounds__(256, 3) void k_sor_march(Dims d, int W, int qt, int
     if (!gate_pass(g, t, tol, status)) return;  // identical in every wave of the grid
   }
   // XCD x (work-group id % 8) takes the x-th eighth of the work-group range;
-  // tiles are numbered k-chunk fastest, so the 4 waves of a work-group take
-  // neighbouring k-chunks of one column and share their halo planes
+  // tiles are numbered x-tile fastest, so the 4 waves of a work-group are
+  // x-neighbours in one k-chunk and row segment and share the cache lines at
+  // their edges; the k-neighbours (sharing halo planes) follow qt tiles on
   const long long per = (long long)gridDim.x / kXcds;  // the grid is a multiple of 8
   const long long blk = (long long)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
   if (blk == 0 && threadIdx.x == 0) *cur = (unsigned)((t + 1) & 1);
@@ -109,7 +112,7 @@ __global__ __launch_bounds__(256, 3) void k_sor_march(Dims d, int W, int qt, int
   {
     // this wave's tables (planes clamped into the domain; a tile past the
     // end fills harmless values it never reads)
-    const int k0t = 1 + (int)(tile % nkc) * K;
+    const int k0t = 1 + (int)((tile / qt) % nkc) * K;
     for (int e = lane; e < NR * 64; e += 64) {
       const int p = e >> 6, mm = e & 63;
       int k = k0t - 1 + p;
@@ -131,10 +134,10 @@ __global__ __launch_bounds__(256, 3) void k_sor_march(Dims d, int W, int qt, int
   float dl = 0.f;
   bool bad = false;
   if (tile < ntiles) {  // wave-uniform, and no barrier inside
-    const int tk = (int)(tile % nkc);
-    const long long tr = tile / nkc;
-    const int tq = (int)(tr % qt);
-    const int tj = (int)(tr / qt);
+    const int tq = (int)(tile % qt);
+    const long long tr = tile / qt;
+    const int tk = (int)(tr % nkc);
+    const int tj = (int)(tr / nkc);
     const int k0 = 1 + tk * K;
     const int k1 = k0 + K < d.NZ - 1 ? k0 + K : d.NZ - 1;  // owned planes [k0, k1)
     const int j0 = 1 + tj * L;
@@ -289,6 +292,8 @@ using namespace qesmc;
 
 namespace {
 
+constexpr int kMarchWavesPerCu = 8;  // segment rule below
+
 // The march's tiling: lanes per tile, k planes, rows per segment.
 struct MarchTiling {
   int K, W, qt, nkc, L;
@@ -303,10 +308,12 @@ MarchTiling march_tiling(const Dims &d, int kplanes, int n_cu)
   m.qt = (d.HP + 61) / 62;
   m.W = (d.HP + m.qt - 1) / m.qt;
   m.nkc = (d.NZ - 2 + m.K - 1) / m.K;
-  // rows per segment: the most (at most 64) that still give >= 16 waves per CU
+  // rows per segment: the most (at most 64) that still give >= 8 wave tiles
+  // per CU (measured: 16 gave 4-row segments on 400 x 400 x 120, 72.7 us per
+  // iteration against 64 us with 8-row ones; 1000 x 1000 x 150 within 1 %)
   auto tiles = [&](int l) { return (long long)m.qt * m.nkc * ((d.NY - 2 + l - 1) / l); };
   m.L = 64;
-  while (m.L > 4 && tiles(m.L) < 16LL * n_cu) m.L /= 2;
+  while (m.L > 4 && tiles(m.L) < (long long)kMarchWavesPerCu * n_cu) m.L /= 2;
   m.ntiles = tiles(m.L);
   m.blocks = (m.ntiles + kWaves - 1) / kWaves;
   m.blocks = (m.blocks + kXcds - 1) / kXcds * kXcds;

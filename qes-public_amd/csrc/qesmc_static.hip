@@ -75,7 +75,9 @@ int qesmc_run_static(qesmc_ctx *x, const qesmc_params *pp, qesmc_result *res)
   const int own_cap = x->err_cap;
   x->d_err = rows;
   x->err_cap = kStaticMaxIter + 1;
+  x->no_alloc_run = true;
   rc = qesmc::run_solve(x, pp, res);
+  x->no_alloc_run = false;
   x->d_err = own;
   x->err_cap = own_cap;
   return rc;

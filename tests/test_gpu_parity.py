@@ -14,7 +14,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-PLAIN, SPLIT, FUSED = 1, 2, 3
+AUTO, PLAIN, SPLIT, FUSED = 0, 1, 2, 3
 
 
 def single_cube_city(O):
@@ -217,8 +217,10 @@ def test_city_400x400x120_reference_schedule(gpu_ctx, oracle, qesmc):
     u_ref = 0.5 / 0.4 * math.log((10.0 + 0.5) / 0.5)
     p = oracle.build_city(ncx, ncy, dx, dz, blds, (1, u_ref, 10.0, 0.5, 217.0))
     ou, ov, ow, olam, oit, ome = oracle.solve_rb_f32(p, itermax=500)
-    for path in (FUSED, SPLIT, PLAIN):
+    for path in (FUSED, SPLIT, PLAIN, AUTO):
         u, v, w, lam, res = gpu_ctx.solve(p, path=path)
+        # the automatic choice: these arrays fit the Infinity Cache
+        assert res["path"] == (SPLIT if path == AUTO else path)
         assert res["iterations"] == oit == 500
         assert_same(lam, olam, f"lambda path {path}")
         assert_same(u, ou, "u")
@@ -239,14 +241,24 @@ def test_city_1000x1000x150_bit_exact(oracle, qesmc):
     assert p.ncell == 1000 * 1000 * 151
     ou, ov, ow, olam, oit, ome = oracle.solve_rb_f32(p, itermax=12)
     with qesmc.Context(0) as ctx:
-        for path, slab in ((SPLIT, 0), (SPLIT, -1), (SPLIT, 20), (FUSED, 0), (PLAIN, 0)):
+        for path, slab in ((SPLIT, 0), (SPLIT, -1), (SPLIT, 20), (FUSED, 0), (PLAIN, 0), (AUTO, 0)):
             u, v, w, lam, res = ctx.solve(p, path=path, itermax=12, slab_planes=slab)
-            assert res["iterations"] == oit == 12 and res["path"] == path
+            # the automatic choice: the two-colour march beyond the Infinity Cache
+            assert res["iterations"] == oit == 12 and res["path"] == (FUSED if path == AUTO else path)
             assert np.float32(res["final_max_dlambda"]) == np.float32(ome)
             assert_same(lam, olam, f"lambda path {path} slab {slab}")
             assert_same(u, ou, "u")
             assert_same(v, ov, "v")
             assert_same(w, ow, "w")
+    # a no-allocation run chooses the march only once its buffers exist
+    with qesmc.Context(0) as ctx:
+        ctx.set_geometry(p)
+        ctx.set_inflow(p.u0, p.v0, p.w0)
+        assert ctx.run_static(itermax=2, tol=-1.0)["path"] == SPLIT
+        assert ctx.run(itermax=2, tol=-1.0)["path"] == FUSED
+        assert ctx.run_static(itermax=2, tol=-1.0)["path"] == FUSED
+        # and refinement runs on the split kernel
+        assert ctx.run(itermax=2, tol=-1.0, refine_every=1)["path"] == SPLIT
 
 
 @pytest.mark.parametrize("tile", [(1, 1, 1), (3, 1, 2), (8, 2, 8), (16, 4, 4), (5, 4, 16), (64, 2, 1)])
