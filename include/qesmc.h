@@ -70,8 +70,8 @@ typedef struct qesmc_fields {
 /* Kernel path selection (qesmc_params.path). */
 #define QESMC_PATH_AUTO 0     /* fastest available: FUSED where the colour
                                  arrays exceed the 256 MB Infinity Cache and
-                                 the run allows it (wall mask, no refinement
-                                 or schedule, below 2^31 half-cells, its
+                                 the run allows it (wall mask, no Chebyshev
+                                 schedule, below 2^31 half-cells, its
                                  buffers already allocated in a
                                  qesmc_run_static call), SPLIT otherwise        */
 #define QESMC_PATH_PLAIN 1    /* in-place red/black launches on the x-fastest
@@ -109,7 +109,7 @@ typedef struct qesmc_params {
                         fixes the end at t* + ceil(stop_extend * t*); the
                         iterations up to it run whatever max|dlambda| does.
                         Decided by the device gate, like the reference's.   */
-  int refine_every; /* 0: off.  K > 0 (split path only): the run iterates on a
+  int refine_every; /* 0: off.  K > 0 (split and fused paths): the run iterates on a
                        correction delta of lambda = base + delta, and every K
                        iterations folds delta into base and forms the
                        residual of base in FP64 (as qesmc_refine does), so
@@ -368,8 +368,9 @@ int qesmc_estimate_omega(qesmc_ctx *ctx, int steps, double *rho, float *omega);
  * the fp32 rounding of lambda into a noise floor of the velocity field
  * (about 1e-4 m/s on a 1000 x 1000 x 150 grid at omega 1.99); refinement
  * takes the field below it.  From lambda = 0 the result is qesmc_run's,
- * bit for bit.  Split path only (QESMC_PATH_PLAIN or _FUSED:
- * QESMC_ERR_UNSUPPORTED).  In res, ms_total includes the residual. */
+ * bit for bit.  Split and fused paths, with the same values on both
+ * (QESMC_PATH_PLAIN: QESMC_ERR_UNSUPPORTED).  In res, ms_total includes the
+ * residual. */
 int qesmc_refine(qesmc_ctx *ctx, const qesmc_params *params, qesmc_result *res);
 
 /* ---- ensembles on one geometry (no counterpart in the reference) ----------

@@ -87,6 +87,7 @@ void free_geometry(qesmc_ctx *x)
     dfree(x->d_mask[c]);
     dfree(x->d_lc2[c]);
     dfree(x->d_R0[c]);
+    dfree(x->d_base[c]);
     for (auto &p : x->d_cc[c]) dfree(p);
   }
   dfree(x->d_canon);
@@ -375,7 +376,10 @@ int run_prepare(qesmc_ctx *x, const qesmc_params *pp, RunState &st)
   if (p.refine_every < 0) return QESMC_ERR_ARG;
   if (!(p.chebyshev_rho >= 0.0 && p.chebyshev_rho < 1.0) || p.chebyshev_start < 0) return QESMC_ERR_ARG;
   st.path = (p.path == QESMC_PATH_PLAIN || p.path == QESMC_PATH_FUSED) ? p.path : QESMC_PATH_SPLIT;
-  if ((p.refine_every > 0 || p.chebyshev_rho > 0.0) && st.path != QESMC_PATH_SPLIT) return QESMC_ERR_UNSUPPORTED;
+  // the schedule's factor per colour pass needs one launch per colour; the
+  // refinement runs on the split kernel and on the march
+  if (p.chebyshev_rho > 0.0 && st.path != QESMC_PATH_SPLIT) return QESMC_ERR_UNSUPPORTED;
+  if (p.refine_every > 0 && st.path == QESMC_PATH_PLAIN) return QESMC_ERR_UNSUPPORTED;
   HIPCHK(hipSetDevice(x->device));
   if (p.flags & QESMC_RUN_REENCODE) {
     int rc = encode_stencil(x);
@@ -385,11 +389,11 @@ int run_prepare(qesmc_ctx *x, const qesmc_params *pp, RunState &st)
   // QESMC_PATH_AUTO: the two-colour march where the colour arrays (lambda and
   // R of both colours, the masks) exceed MI355X's 256 MB Infinity Cache, so
   // that each iteration streams them from HBM, and where it applies (wall
-  // mask, no refinement or schedule, 32-bit offsets, its buffers available
-  // without allocating in a no-allocation run); the split kernel otherwise.
+  // mask, no schedule, 32-bit offsets, its buffers available without
+  // allocating in a no-allocation run); the split kernel otherwise.
   // Measured: 568 against 626 us per iteration on 1000 x 1000 x 150; 64
   // against 58 us on 400 x 400 x 120, whose arrays stay in the cache.
-  if (p.path == QESMC_PATH_AUTO && st.masked && p.refine_every == 0 && p.chebyshev_rho == 0.0 &&
+  if (p.path == QESMC_PATH_AUTO && st.masked && p.chebyshev_rho == 0.0 &&
       x->nsplit < (1LL << 31) && (double)x->nsplit * 2.0 * (4.0 + 4.0 + 1.0) > 256.0 * (1 << 20) &&
       (!x->no_alloc_run || (x->d_lc2[0] && x->d_lc2[1] && x->d_cur && x->d_canon8)))
     st.path = QESMC_PATH_FUSED;
@@ -454,7 +458,7 @@ int run_prepare(qesmc_ctx *x, const qesmc_params *pp, RunState &st)
   if (p.refine_every > 0) {
     // base lambda and the run's R, besides the correction and the residual
     for (int c = 0; c < 2; ++c) {
-      if (!x->d_lc2[c]) HIPCHK(dalloc(&x->d_lc2[c], x->nsplit));
+      if (!x->d_base[c]) HIPCHK(dalloc(&x->d_base[c], x->nsplit));
       if (!x->d_R0[c]) HIPCHK(dalloc(&x->d_R0[c], x->nsplit));
     }
   }
@@ -489,7 +493,7 @@ int run_prepare(qesmc_ctx *x, const qesmc_params *pp, RunState &st)
     HIPCHK(qesmc_launch_set_u32(x, x->d_cur, 0u));
   }
   HIPCHK(qesmc_launch_divergence(x, st.path != QESMC_PATH_PLAIN, p.alpha1));
-  if (p.refine_every > 0) HIPCHK(refine_setup(x, st.masked));
+  if (p.refine_every > 0) HIPCHK(refine_setup(x, st.masked, st.path == QESMC_PATH_FUSED));
   HIPCHK(hipEventRecord(x->ev[1], s));
   st.first_runs = p.itermax > 0 && (1.0f > p.tol);
   return QESMC_OK;
@@ -596,7 +600,7 @@ int run_loop(qesmc_ctx *x, const RunState &st, int *launched_out)
       }
       ++launched;
       if (p.refine_every > 0 && (t + 1) % p.refine_every == 0 && t + 1 < p.itermax &&
-          (e = refine_refresh(x, st.masked)) != hipSuccess) {
+          (e = refine_refresh(x, st.masked, st.path == QESMC_PATH_FUSED)) != hipSuccess) {
         rc = map_hip(e);
         break;
       }
@@ -642,7 +646,7 @@ int run_solve(qesmc_ctx *x, const qesmc_params *pp, qesmc_result *res)
   rc = run_loop(x, st, &launched);
   if (rc != QESMC_OK) return rc;
   if (st.p.refine_every > 0) {
-    hipError_t e = refine_fold_final(x);
+    hipError_t e = refine_fold_final(x, st.path == QESMC_PATH_FUSED);
     if (e != hipSuccess) return map_hip(e);
   }
   return qesmc::run_finish(x, st, launched, res);

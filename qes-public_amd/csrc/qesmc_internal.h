@@ -129,6 +129,7 @@ struct qesmc_ctx {
   float *d_lc2[2] = {};       // second lambda buffer set of the fused path (allocated on first use)
   unsigned *d_cur = nullptr;   // fused path: buffer set holding the latest iterate (0: d_lc, 1: d_lc2)
   float *d_R0[2] = {};        // refine_every: R of the run (d_Rc then holds the residual of the base)
+  float *d_base[2] = {};      // refine_every: the base lambda (the iterates hold the correction)
 
   // convergence slots, kErrSlots words per iteration: row 0 holds the 1.0f
   // sentinel, row t+1 the striped max|dlambda| of iteration t
@@ -164,9 +165,11 @@ int run_loop(qesmc_ctx *x, const RunState &st, int *launched);
 // refine_every: set up base = lambda, R0 = R, R = residual of base, delta
 // = 0 (refine_setup); fold delta into base and re-form the residual
 // (refine_refresh); lambda = base + delta at the end (refine_fold_final).
-hipError_t refine_setup(qesmc_ctx *x, bool masked);
-hipError_t refine_refresh(qesmc_ctx *x, bool masked);
-hipError_t refine_fold_final(qesmc_ctx *x);
+// With `fused` the correction lives in whichever buffer set the march wrote
+// last (d_cur, read on the device); otherwise in d_lc.
+hipError_t refine_setup(qesmc_ctx *x, bool masked, bool fused);
+hipError_t refine_refresh(qesmc_ctx *x, bool masked, bool fused);
+hipError_t refine_fold_final(qesmc_ctx *x, bool fused);
 // Whether the resident flags hold a cut cell (7 or 8).
 hipError_t find_cut_cells(qesmc_ctx *x, bool *cut);
 int static_scratch_init(int device);

@@ -81,13 +81,18 @@ __global__ __launch_bounds__(256) void k_refine_residual(Dims d, int colour, con
   }
 }
 
-// lam += base over one colour array (every position: the ring and the
-// planes that are never relaxed hold zero in the correction).
-__global__ __launch_bounds__(256) void k_refine_add(long long n, float *__restrict__ lam,
-                                                    const float *__restrict__ base)
+// dst += src over one colour array (every position: the ring and the
+// planes that are never relaxed hold zero in the correction).  With cur the
+// buffer set *cur selects (dst1 / src1 for set 1) stands for the one the
+// two-colour march wrote last; without it set 0.
+__global__ __launch_bounds__(256) void k_refine_add(long long n, float *dst0, float *dst1, const float *src0,
+                                                    const float *src1, const unsigned *__restrict__ cur)
 {
+  const bool one = cur != nullptr && *cur != 0u;
+  float *__restrict__ dst = one ? dst1 : dst0;
+  const float *__restrict__ src = one ? src1 : src0;
   const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long x = blockIdx.x * 256LL + threadIdx.x; x < n; x += stride) lam[x] = base[x] + lam[x];
+  for (long long x = blockIdx.x * 256LL + threadIdx.x; x < n; x += stride) dst[x] = src[x] + dst[x];
 }
 
 // The k = 0 mirror of a base lambda between refreshes: ghost (i0,j,0) of
@@ -135,21 +140,32 @@ hipError_t residual(qesmc_ctx *x, bool masked, float *const *base, float *const 
   return hipSuccess;
 }
 
-hipError_t zero_correction(qesmc_ctx *x)
+// the correction = 0: d_lc, and with `fused` both of the march's buffer sets
+hipError_t zero_correction(qesmc_ctx *x, bool fused)
 {
-  for (int c = 0; c < 2; ++c) {
-    const hipError_t e = qesmc_launch_fill_u32(x, reinterpret_cast<unsigned *>(x->d_lc[c]), x->nsplit, 0u);
-    if (e != hipSuccess) return e;
-  }
+  for (int set = 0; set < (fused ? 2 : 1); ++set)
+    for (int c = 0; c < 2; ++c) {
+      float *a = set ? x->d_lc2[c] : x->d_lc[c];
+      const hipError_t e = qesmc_launch_fill_u32(x, reinterpret_cast<unsigned *>(a), x->nsplit, 0u);
+      if (e != hipSuccess) return e;
+    }
   return hipSuccess;
 }
 
-// dst = dst + src over both colour arrays
-hipError_t add(qesmc_ctx *x, float *const *dst, float *const *src)
+// base += correction (to_base) or correction += base, over both colour
+// arrays; the correction is the march's latest buffer set with `fused`
+hipError_t add(qesmc_ctx *x, bool to_base, bool fused)
 {
   const unsigned g = grid_for(x->nsplit);
+  const unsigned *cur = fused ? x->d_cur : nullptr;
   for (int c = 0; c < 2; ++c) {
-    hipLaunchKernelGGL(k_refine_add, dim3(g), dim3(256), 0, x->stream, x->nsplit, dst[c], (const float *)src[c]);
+    float *d0 = x->d_lc[c], *d1 = fused ? x->d_lc2[c] : x->d_lc[c];
+    if (to_base)
+      hipLaunchKernelGGL(k_refine_add, dim3(g), dim3(256), 0, x->stream, x->nsplit, x->d_base[c], x->d_base[c],
+                         (const float *)d0, (const float *)d1, cur);
+    else
+      hipLaunchKernelGGL(k_refine_add, dim3(g), dim3(256), 0, x->stream, x->nsplit, d0, d1,
+                         (const float *)x->d_base[c], (const float *)x->d_base[c], cur);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
@@ -160,37 +176,39 @@ hipError_t add(qesmc_ctx *x, float *const *dst, float *const *src)
 
 namespace qesmc {
 
-// After R: base (d_lc2) = lambda, R0 = R, R = residual of base, delta (d_lc) = 0.
-hipError_t refine_setup(qesmc_ctx *x, bool masked)
+// After R: base = lambda (d_lc; the march's second set holds the same), R0
+// = R, R = residual of base, delta = 0.
+hipError_t refine_setup(qesmc_ctx *x, bool masked, bool fused)
 {
   const size_t bytes = sizeof(float) * (size_t)x->nsplit;
   for (int c = 0; c < 2; ++c) {
     hipError_t e = hipMemcpyAsync(x->d_R0[c], x->d_Rc[c], bytes, hipMemcpyDeviceToDevice, x->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(x->d_lc2[c], x->d_lc[c], bytes, hipMemcpyDeviceToDevice, x->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(x->d_base[c], x->d_lc[c], bytes, hipMemcpyDeviceToDevice, x->stream);
     if (e != hipSuccess) return e;
   }
-  hipError_t e = residual(x, masked, x->d_lc2, x->d_R0);
-  return e == hipSuccess ? zero_correction(x) : e;
+  hipError_t e = residual(x, masked, x->d_base, x->d_R0);
+  return e == hipSuccess ? zero_correction(x, fused) : e;
 }
 
 // Between iterations: base += delta, the base's k = 0 mirror, R = residual
 // of base, delta = 0.
-hipError_t refine_refresh(qesmc_ctx *x, bool masked)
+hipError_t refine_refresh(qesmc_ctx *x, bool masked, bool fused)
 {
-  hipError_t e = add(x, x->d_lc2, x->d_lc);
+  hipError_t e = add(x, true, fused);
   if (e != hipSuccess) return e;
   const unsigned g = grid_for(x->d.splane);
   for (int c = 0; c < 2; ++c) {
-    hipLaunchKernelGGL(k_refine_mirror, dim3(g), dim3(256), 0, x->stream, x->d, c, x->d_lc2[c],
-                       (const float *)x->d_lc2[c ^ 1]);
+    hipLaunchKernelGGL(k_refine_mirror, dim3(g), dim3(256), 0, x->stream, x->d, c, x->d_base[c],
+                       (const float *)x->d_base[c ^ 1]);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
-  e = residual(x, masked, x->d_lc2, x->d_R0);
-  return e == hipSuccess ? zero_correction(x) : e;
+  e = residual(x, masked, x->d_base, x->d_R0);
+  return e == hipSuccess ? zero_correction(x, fused) : e;
 }
 
-// lambda = base + delta.
-hipError_t refine_fold_final(qesmc_ctx *x) { return add(x, x->d_lc, x->d_lc2); }
+// lambda = base + delta, in the set that holds delta (the one the unpacking
+// reads).
+hipError_t refine_fold_final(qesmc_ctx *x, bool fused) { return add(x, false, fused); }
 
 }  // namespace qesmc
 
@@ -200,7 +218,7 @@ int qesmc_refine(qesmc_ctx *x, const qesmc_params *pp, qesmc_result *res)
 {
   if (!x || !pp) return QESMC_ERR_ARG;
   qesmc_params p = *pp;
-  if (p.path == QESMC_PATH_PLAIN || p.path == QESMC_PATH_FUSED) return QESMC_ERR_UNSUPPORTED;
+  if (p.path == QESMC_PATH_PLAIN) return QESMC_ERR_UNSUPPORTED;
   if (p.refine_every < 0) return QESMC_ERR_ARG;
   // the resident lambda is the base; no refresh inside unless asked for
   if (p.refine_every == 0) p.refine_every = INT_MAX;

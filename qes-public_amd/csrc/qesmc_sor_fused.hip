@@ -109,10 +109,15 @@ __global__ __launch_bounds__(256, 3) void k_sor_march(Dims d, int W, int qt, int
   const long long blk = (long long)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
   if (blk == 0 && threadIdx.x == 0) *cur = (unsigned)((t + 1) & 1);
   const long long tile = blk * kWaves + wave;
+  const int tq = (int)(tile % qt);
+  const long long tr = tile / qt;
+  const int tk = (int)(tr % nkc);
+  const long long tj = tr / nkc;
+  const bool valid = tile < ntiles;
   {
     // this wave's tables (planes clamped into the domain; a tile past the
     // end fills harmless values it never reads)
-    const int k0t = 1 + (int)((tile / qt) % nkc) * K;
+    const int k0t = 1 + tk * K;
     for (int e = lane; e < NR * 64; e += 64) {
       const int p = e >> 6, mm = e & 63;
       int k = k0t - 1 + p;
@@ -133,14 +138,10 @@ __global__ __launch_bounds__(256, 3) void k_sor_march(Dims d, int W, int qt, int
 
   float dl = 0.f;
   bool bad = false;
-  if (tile < ntiles) {  // wave-uniform, and no barrier inside
-    const int tq = (int)(tile % qt);
-    const long long tr = tile / qt;
-    const int tk = (int)(tr % nkc);
-    const int tj = (int)(tr / nkc);
+  if (valid) {  // wave-uniform, and no barrier inside
     const int k0 = 1 + tk * K;
     const int k1 = k0 + K < d.NZ - 1 ? k0 + K : d.NZ - 1;  // owned planes [k0, k1)
-    const int j0 = 1 + tj * L;
+    const int j0 = 1 + (int)tj * L;
     const int j1 = j0 + L < d.NY - 1 ? j0 + L : d.NY - 1;  // owned rows [j0, j1)
     // lanes past the right halo (W + 2 <= 64) repeat its address: they load
     // nothing the wave does not already fetch, and own nothing

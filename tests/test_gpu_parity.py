@@ -257,8 +257,13 @@ def test_city_1000x1000x150_bit_exact(oracle, qesmc):
         assert ctx.run_static(itermax=2, tol=-1.0)["path"] == SPLIT
         assert ctx.run(itermax=2, tol=-1.0)["path"] == FUSED
         assert ctx.run_static(itermax=2, tol=-1.0)["path"] == FUSED
-        # and refinement runs on the split kernel
-        assert ctx.run(itermax=2, tol=-1.0, refine_every=1)["path"] == SPLIT
+        # refinement runs on the march too, with the split kernel's values
+        ctx.set_lambda(None)
+        assert ctx.run(itermax=3, tol=-1.0, refine_every=2)["path"] == FUSED
+        lam = ctx.get_lambda()
+        ctx.set_lambda(None)
+        assert ctx.run(itermax=3, tol=-1.0, refine_every=2, path=SPLIT)["path"] == SPLIT
+        assert_same(ctx.get_lambda(), lam, "refined lambda, march against split")
 
 
 @pytest.mark.parametrize("tile", [(1, 1, 1), (3, 1, 2), (8, 2, 8), (16, 4, 4), (5, 4, 16), (64, 2, 1)])

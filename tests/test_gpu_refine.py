@@ -39,21 +39,23 @@ def _linf(a, b):
     return float(np.abs(a.astype(np.float64) - b).max())
 
 
-@pytest.mark.parametrize("full", [False, True])
-def test_refine_from_zero_is_the_plain_run(qesmc, oracle, full):
+@pytest.mark.parametrize("full,fused", [(False, False), (True, False), (False, True)])
+def test_refine_from_zero_is_the_plain_run(qesmc, oracle, full, fused):
     p = _city(oracle, qesmc, 21, 48, 40, 14, True)
     if full:
         rng = np.random.default_rng(5)
         for a in p.coeffs:
             a *= rng.uniform(0.3, 1.0, a.size).astype(np.float32)
+    path = qesmc.PATH_FUSED if fused else qesmc.PATH_SPLIT
     with qesmc.Context(0) as ctx:
         _load(ctx, p)
         ctx.set_lambda(None)
-        a = ctx.run(itermax=37, tol=-1.0, omega=1.9)
+        a = ctx.run(itermax=37, tol=-1.0, omega=1.9, path=path)
         ref = ctx.get_lambda(), ctx.get_velocity()
         ctx.set_lambda(None)
-        b = ctx.refine(itermax=37, tol=-1.0, omega=1.9)
+        b = ctx.refine(itermax=37, tol=-1.0, omega=1.9, path=path)
         got = ctx.get_lambda(), ctx.get_velocity()
+    assert a["path"] == b["path"] == path
     assert a["compressed"] == (0 if full else 1) and b["compressed"] == a["compressed"]
     assert a["iterations"] == b["iterations"] == 37
     assert np.array_equal(ref[0], got[0])
@@ -119,27 +121,47 @@ def test_refine_goes_below_the_fp32_noise_floor(qesmc, oracle):
     assert max(ref) * 4 <= floor
 
 
-@pytest.mark.parametrize("full", [False, True])
-def test_refine_every_equals_chained_refinements(qesmc, oracle, full):
+@pytest.mark.parametrize("full,fused", [(False, False), (True, False), (False, True)])
+def test_refine_every_equals_chained_refinements(qesmc, oracle, full, fused):
     """A run that refreshes its base every K iterations equals K-iteration
     qesmc_refine calls one after another (each folds, mirrors k = 0 and forms
     the residual in between), bit for bit; the k = 0 ghosts of the base must
-    be re-mirrored at each refresh for this to hold."""
+    be re-mirrored at each refresh for this to hold.  On the two-colour march
+    (whose correction alternates between two buffer sets: after 23 and 69
+    iterations it is in the second, after 46 in the first) the same, and the
+    same values as on the split kernel."""
     p = _city(oracle, qesmc, 24, 44, 38, 13, True)
     if full:
         rng = np.random.default_rng(6)
         for a in p.coeffs:
             a *= rng.uniform(0.3, 1.0, a.size).astype(np.float32)
     K = 23
+    path = qesmc.PATH_FUSED if fused else qesmc.PATH_SPLIT
     with qesmc.Context(0) as ctx:
         _load(ctx, p)
         ctx.set_lambda(None)
         for _ in range(3):
-            ctx.refine(itermax=K, tol=-1.0, omega=1.93)
+            ctx.refine(itermax=K, tol=-1.0, omega=1.93, path=path)
         ref = ctx.get_lambda(), ctx.get_velocity()
         ctx.set_lambda(None)
-        r = ctx.run(itermax=3 * K, tol=-1.0, omega=1.93, refine_every=K)
+        r = ctx.run(itermax=3 * K, tol=-1.0, omega=1.93, refine_every=K, path=path)
         got = ctx.get_lambda(), ctx.get_velocity()
+        assert r["path"] == path
+        if fused:
+            # the split kernel's refined run: the same relaxations, so the same values
+            ctx.set_lambda(None)
+            ctx.run(itermax=3 * K, tol=-1.0, omega=1.93, refine_every=K, path=qesmc.PATH_SPLIT)
+            assert np.array_equal(ctx.get_lambda(), got[0])
+            # a refresh after a stop: the device's stop rule ends the march
+            # between refreshes, and the correction is folded from the set it
+            # wrote last
+            ctx.set_lambda(None)
+            a = ctx.run(itermax=500, tol=2e-3, omega=1.93, refine_every=K, path=path)
+            fl = ctx.get_lambda()
+            ctx.set_lambda(None)
+            b = ctx.run(itermax=500, tol=2e-3, omega=1.93, refine_every=K, path=qesmc.PATH_SPLIT)
+            assert a["iterations"] == b["iterations"] < 500
+            assert np.array_equal(fl, ctx.get_lambda())
         # and not the plain run's (the rounding differs once a base exists)
         ctx.set_lambda(None)
         ctx.run(itermax=3 * K, tol=-1.0, omega=1.93)
@@ -178,13 +200,12 @@ def test_refine_every_refusals(qesmc, oracle):
     p = _city(oracle, qesmc, 25, 30, 26, 10, False)
     with qesmc.Context(0) as ctx:
         _load(ctx, p)
-        for path in (qesmc.PATH_PLAIN, qesmc.PATH_FUSED):
-            with pytest.raises(qesmc.QesmcError) as e:
-                ctx.run(itermax=10, refine_every=5, path=path)
-            assert e.value.code == qesmc.ERR_UNSUPPORTED
-            with pytest.raises(qesmc.QesmcError) as e:
-                ctx.refine(itermax=10, path=path)
-            assert e.value.code == qesmc.ERR_UNSUPPORTED
+        with pytest.raises(qesmc.QesmcError) as e:
+            ctx.run(itermax=10, refine_every=5, path=qesmc.PATH_PLAIN)
+        assert e.value.code == qesmc.ERR_UNSUPPORTED
+        with pytest.raises(qesmc.QesmcError) as e:
+            ctx.refine(itermax=10, path=qesmc.PATH_PLAIN)
+        assert e.value.code == qesmc.ERR_UNSUPPORTED
         with pytest.raises(qesmc.QesmcError) as e:
             ctx.run(itermax=10, refine_every=-1)
         assert e.value.code == qesmc.ERR_ARG
