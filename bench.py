@@ -81,8 +81,8 @@ def bytes_per_lup(res):
       split, wall mask : own lambda r+w 8 + R 4 + mask 1 + other-colour lambda 4 = 17
       split, full coef : 8 + 4 + 24 + 4 = 40
       fused, wall mask : own lambda r+w 8 + R 4 + mask 1 = 13 (each lambda is
-                         read once per iteration: the black pass takes its red
-                         neighbours from the work-group's LDS)
+                         read once per iteration: a wave's march keeps the
+                         rows around the ones it relaxes in registers)
       plain            : coefficient and R lines of both colours are fetched by
                          each pass: 2*(24+4) + lambda r (4, neighbours shared) + w 4 = 64"""
     if res["path"] == Q.PATH_PLAIN:
@@ -692,8 +692,9 @@ def c5_tol_leg(tol_ms=1e-4, dirs=8, cap=20000, cpu_ms_per_it=None, omega_steps=1
                    "speedup_vs_reference_cold": round(dirs * cold_ms / ens_ms, 2),
                    "plain_path_cold_ms_extrapolated": round(dirs * cold_its * plain_ms_it, 1),
                    "speedup_vs_plain_path_cold": round(dirs * cold_its * plain_ms_it / ens_ms, 2),
-                   "note": "the cold totals are %d x the measured cold solve at omega 1.78 (270 deg) on the split "
-                           "kernel, and at the classic-layout path's measured time per iteration" % dirs}
+                   "note": "the cold totals are %d x the measured cold solve at omega 1.78 (270 deg) on the "
+                           "library's default path (the one every solve here runs on), and at the classic-layout "
+                           "path's measured time per iteration" % dirs}
             if cpu_ms_per_it:
                 ens["cpu_cold_ms_extrapolated"] = round(dirs * cold_its * cpu_ms_per_it, 0)
                 ens["speedup_vs_cpu"] = round(dirs * cold_its * cpu_ms_per_it / ens_ms, 1)
