@@ -70,8 +70,8 @@ typedef struct qesmc_fields {
 /* Kernel path selection (qesmc_params.path). */
 #define QESMC_PATH_AUTO 0     /* fastest available: FUSED where the colour
                                  arrays exceed the 256 MB Infinity Cache and
-                                 the run allows it (wall mask, no Chebyshev
-                                 schedule, below 2^31 half-cells, its
+                                 the run allows it (wall mask, below 2^31
+                                 half-cells, its
                                  buffers already allocated in a
                                  qesmc_run_static call), SPLIT otherwise        */
 #define QESMC_PATH_PLAIN 1    /* in-place red/black launches on the x-fastest
@@ -119,8 +119,8 @@ typedef struct qesmc_params {
                        In exact arithmetic the iterates are the plain run's.
                        Not in qesmc_run_batch or qesmc_run_static
                        (QESMC_ERR_UNSUPPORTED). */
-  double chebyshev_rho; /* 0: every colour pass at omega.  In (0, 1) (split
-                       path only): the Jacobi spectral radius rho_J (e.g.
+  double chebyshev_rho; /* 0: every colour pass at omega.  In (0, 1): the
+                       Jacobi spectral radius rho_J (e.g.
                        from qesmc_estimate_omega), and omega varies by colour
                        pass as cyclic Chebyshev semi-iteration prescribes
                        (Golub & Varga 1961): pass h = 0, 1, 2, ... of the

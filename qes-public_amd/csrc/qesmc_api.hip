@@ -376,9 +376,7 @@ int run_prepare(qesmc_ctx *x, const qesmc_params *pp, RunState &st)
   if (p.refine_every < 0) return QESMC_ERR_ARG;
   if (!(p.chebyshev_rho >= 0.0 && p.chebyshev_rho < 1.0) || p.chebyshev_start < 0) return QESMC_ERR_ARG;
   st.path = (p.path == QESMC_PATH_PLAIN || p.path == QESMC_PATH_FUSED) ? p.path : QESMC_PATH_SPLIT;
-  // the schedule's factor per colour pass needs one launch per colour; the
-  // refinement runs on the split kernel and on the march
-  if (p.chebyshev_rho > 0.0 && st.path != QESMC_PATH_SPLIT) return QESMC_ERR_UNSUPPORTED;
+  // the refinement runs on the split kernel and on the march
   if (p.refine_every > 0 && st.path == QESMC_PATH_PLAIN) return QESMC_ERR_UNSUPPORTED;
   HIPCHK(hipSetDevice(x->device));
   if (p.flags & QESMC_RUN_REENCODE) {
@@ -389,11 +387,11 @@ int run_prepare(qesmc_ctx *x, const qesmc_params *pp, RunState &st)
   // QESMC_PATH_AUTO: the two-colour march where the colour arrays (lambda and
   // R of both colours, the masks) exceed MI355X's 256 MB Infinity Cache, so
   // that each iteration streams them from HBM, and where it applies (wall
-  // mask, no schedule, 32-bit offsets, its buffers available without
-  // allocating in a no-allocation run); the split kernel otherwise.
+  // mask, 32-bit offsets, its buffers available without allocating in a
+  // no-allocation run); the split kernel otherwise.
   // Measured: 568 against 626 us per iteration on 1000 x 1000 x 150; 64
   // against 58 us on 400 x 400 x 120, whose arrays stay in the cache.
-  if (p.path == QESMC_PATH_AUTO && st.masked && p.chebyshev_rho == 0.0 &&
+  if (p.path == QESMC_PATH_AUTO && st.masked &&
       x->nsplit < (1LL << 31) && (double)x->nsplit * 2.0 * (4.0 + 4.0 + 1.0) > 256.0 * (1 << 20) &&
       (!x->no_alloc_run || (x->d_lc2[0] && x->d_lc2[1] && x->d_cur && x->d_canon8)))
     st.path = QESMC_PATH_FUSED;

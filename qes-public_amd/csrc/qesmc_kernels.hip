@@ -464,7 +464,8 @@ static const unsigned kPlaneBlocks = 8;  // work-groups per plane of the plane r
 
 using namespace qesmc;
 
-hipError_t qesmc_launch_fused_iteration(qesmc_ctx *x, int t, int itermax, float omega, float tol);
+hipError_t qesmc_launch_fused_iteration(qesmc_ctx *x, int t, int itermax, float omega_red, float omega_black,
+                                        float tol);
 hipError_t qesmc_launch_split_iteration(qesmc_ctx *x, bool masked, int t, int itermax, float omega_red,
                                         float omega_black, float tol);
 
@@ -528,24 +529,23 @@ hipError_t qesmc_launch_unpack_selected(qesmc_ctx *x)
   return hipGetLastError();
 }
 
-// omega_black applies to the split path's black pass; the other paths run
-// both colours at omega_red (the run refuses a schedule there).
+// Iteration t: the red pass at omega_red, the black pass at omega_black.
 hipError_t qesmc_launch_iteration(qesmc_ctx *x, int path, bool masked, int t, int itermax,
                                   float omega_red, float omega_black, float tol)
 {
-  const float omega = omega_red;
   const Dims &d = x->d;
   if (d.NZ - 2 < 1 || d.NY - 2 < 1) return hipSuccess;
-  const float omo = 1.0f - omega;
   const int kp = x->kplanes < 1 ? 1 : x->kplanes;
   const dim3 gr = grid_xy((d.NX + 1) / 2, d.NY - 2, (d.NZ - 2 + kp - 1) / kp);
   if (path == QESMC_PATH_PLAIN) {
-    for (int colour = 0; colour < 2; ++colour)
+    for (int colour = 0; colour < 2; ++colour) {
+      const float omega = colour ? omega_black : omega_red;
       hipLaunchKernelGGL(k_rb_plain, gr, kBlock, 0, x->stream, d, colour, kp, x->d_coef[0],
                          x->d_coef[1], x->d_coef[2], x->d_coef[3], x->d_coef[4], x->d_coef[5],
-                         x->d_R, x->d_lam, omega, omo, tol, x->d_err, t, itermax, x->d_status);
+                         x->d_R, x->d_lam, omega, 1.0f - omega, tol, x->d_err, t, itermax, x->d_status);
+    }
   } else if (path == QESMC_PATH_FUSED) {
-    return qesmc_launch_fused_iteration(x, t, itermax, omega, tol);
+    return qesmc_launch_fused_iteration(x, t, itermax, omega_red, omega_black, tol);
   } else {
     return qesmc_launch_split_iteration(x, masked, t, itermax, omega_red, omega_black, tol);
   }

@@ -82,16 +82,18 @@ __global__ __launch_bounds__(256, 3) void k_sor_march(Dims d, int W, int qt, int
                                                    const float *__restrict__ R0, const float *__restrict__ R1,
                                                    const unsigned char *__restrict__ mask0,
                                                    const unsigned char *__restrict__ mask1,
-                                                   const unsigned *__restrict__ canon8, float omega,
-                                                   float one_m_omega, float tol, unsigned *err, int t,
+                                                   const unsigned *__restrict__ canon8, float omega_r,
+                                                   float omo_r, float omega_b, float omo_b, float tol, unsigned *err,
+                                                   int t,
                                                    int itermax, unsigned *status, unsigned *cur)
 {
   constexpr int NB = K + 4;  // black-old plane slots: planes k0-2 .. k0+K+1
   constexpr int NR = K + 2;  // red plane slots: planes k0-1 .. k0+K
   __shared__ float wmax[kWaves];
-  // per wave and relaxed plane k0-1+p: omega / (sum of the wall values a
-  // cell's mask selects), for each of the 64 masks, and the six wall values
-  __shared__ float wd_tab[kWaves][NR][64];
+  // per wave, colour (red, black) and relaxed plane k0-1+p: omega / (sum of
+  // the wall values a cell's mask selects), for each of the 64 masks, and
+  // the six wall values
+  __shared__ float wd_tab[kWaves][2][NR][64];
   __shared__ unsigned cv_tab[kWaves][NR][8];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -125,7 +127,9 @@ __global__ __launch_bounds__(256, 3) void k_sor_march(Dims d, int W, int qt, int
       float v[6];
 #pragma unroll
       for (int a = 0; a < 6; ++a) v[a] = ((mm >> a) & 1) ? __uint_as_float(canon8[k * 8 + a]) : 0.f;
-      wd_tab[wave][p][mm] = omega / (v[0] + v[1] + v[2] + v[3] + v[4] + v[5]);
+      const float sum = v[0] + v[1] + v[2] + v[3] + v[4] + v[5];
+      wd_tab[wave][0][p][mm] = omega_r / sum;
+      wd_tab[wave][1][p][mm] = omega_b / sum;
     }
     for (int e = lane; e < NR * 8; e += 64) {
       const int p = e >> 3;
@@ -230,8 +234,8 @@ __global__ __launch_bounds__(256, 3) void k_sor_march(Dims d, int W, int qt, int
           const float cup = __shfl_down(c, 1, 64);  // half-cell q+1
           const float xm = s ? c : cdn;
           const float xp = s ? cup : c;
-          const float nv = relax_tab(rmk[p], cv, wd_tab[wave][p][rmk[p] & 63u], xp, xm, bp[p + 1], bm[p + 1],
-                                     bc[p + 2], bc[p], rr[p], ro[p], one_m_omega);
+          const float nv = relax_tab(rmk[p], cv, wd_tab[wave][0][p][rmk[p] & 63u], xp, xm, bp[p + 1], bm[p + 1],
+                                     bc[p + 2], bc[p], rr[p], ro[p], omo_r);
           rn[p] = upd ? nv : ro[p];
           if (rown && k >= k0 && k < k1) {  // wave-uniform; k >= k0: never the halo plane
             const bool cnt = own && upd;
@@ -253,8 +257,8 @@ __global__ __launch_bounds__(256, 3) void k_sor_march(Dims d, int W, int qt, int
           const float xm = s ? c : cdn;
           const float xp = s ? cup : c;
           const float o = bm[p + 1];
-          const float nv = relax_tab(bmk[p - 1], cv, wd_tab[wave][p][bmk[p - 1] & 63u], xp, xm, rn[p], rm2[p],
-                                     rm1[p + 1], rm1[p - 1], br[p - 1], o, one_m_omega);
+          const float nv = relax_tab(bmk[p - 1], cv, wd_tab[wave][1][p][bmk[p - 1] & 63u], xp, xm, rn[p], rm2[p],
+                                     rm1[p + 1], rm1[p - 1], br[p - 1], o, omo_b);
           const bool cnt = own && upd;
           const float dd = cnt ? fabsf(nv - o) : 0.f;
           dl = (dd > dl) ? dd : dl;
@@ -343,7 +347,8 @@ hipError_t qesmc_fused_prepare(qesmc_ctx *x)
 }
 
 // Launch iteration t of the fused path: reads buffer set t & 1, writes the other.
-hipError_t qesmc_launch_fused_iteration(qesmc_ctx *x, int t, int itermax, float omega, float tol)
+hipError_t qesmc_launch_fused_iteration(qesmc_ctx *x, int t, int itermax, float omega_red, float omega_black,
+                                        float tol)
 {
   const Dims &d = x->d;
   if (d.NZ - 2 < 1 || d.NY - 2 < 1) return hipSuccess;
@@ -354,8 +359,8 @@ hipError_t qesmc_launch_fused_iteration(qesmc_ctx *x, int t, int itermax, float 
 #define QESMC_MARCH_LAUNCH(KK)                                                                                 \
   hipLaunchKernelGGL(k_sor_march<KK>, dim3((unsigned)m.blocks), dim3(64 * kWaves), 0, x->stream, d, m.W, m.qt, \
                      m.L, m.nkc, m.ntiles, A[0], A[1], B[0], B[1], x->d_Rc[0], x->d_Rc[1], x->d_mask[0],        \
-                     x->d_mask[1], x->d_canon8, omega, 1.0f - omega, tol, x->d_err, t, itermax, x->d_status,     \
-                     x->d_cur)
+                     x->d_mask[1], x->d_canon8, omega_red, 1.0f - omega_red, omega_black, 1.0f - omega_black, tol,  \
+                     x->d_err, t, itermax, x->d_status, x->d_cur)
   if (m.K == 8) QESMC_MARCH_LAUNCH(8);
   else QESMC_MARCH_LAUNCH(4);
 #undef QESMC_MARCH_LAUNCH

@@ -2,15 +2,15 @@
 
   (1) lambda after N scheduled iterations from zero equals the oracle's
       red-black loop with the same per-pass factors
-      (oracle.chebyshev_omegas), bit for bit, on the wall-mask and the
-      full-coefficient stencils;
+      (oracle.chebyshev_omegas), bit for bit, on the split kernel (wall-mask
+      and full-coefficient stencils), the two-colour march (a factor per
+      colour within its launch) and the classic layout;
   (2) a solve split over calls that pass the running count
       (chebyshev_start) is the one-call solve, bit for bit, and a run with
       refine_every = K is K-iteration refinements with running starts;
   (3) qesmc_run_static gives qesmc_run's result, and a far running start
       runs at the recurrence's limit, Young's optimum;
-  (4) out-of-range factors and the paths without a per-colour factor are
-      refused."""
+  (4) out-of-range factors and batched runs are refused."""
 import time
 
 import numpy as np
@@ -30,8 +30,8 @@ def _load(ctx, p):
     ctx.set_inflow(p.u0, p.v0, p.w0)
 
 
-@pytest.mark.parametrize("full", [False, True])
-def test_schedule_matches_the_oracle(qesmc, oracle, full):
+@pytest.mark.parametrize("full,path", [(False, 2), (True, 2), (False, 3), (False, 1)])
+def test_schedule_matches_the_oracle(qesmc, oracle, full, path):
     p = _city(oracle, qesmc, 31, 40, 34, 13)
     if full:
         rng = np.random.default_rng(7)
@@ -43,9 +43,11 @@ def test_schedule_matches_the_oracle(qesmc, oracle, full):
     with qesmc.Context(0) as ctx:
         _load(ctx, p)
         ctx.set_lambda(None)
-        r = ctx.run(itermax=N, tol=-1.0, chebyshev_rho=rho)
+        r = ctx.run(itermax=N, tol=-1.0, chebyshev_rho=rho, path=path)
         got = ctx.get_lambda()
-    assert r["compressed"] == (0 if full else 1) and r["iterations"] == N
+    assert r["path"] == path and r["iterations"] == N
+    if path != qesmc.PATH_PLAIN:
+        assert r["compressed"] == (0 if full else 1)
     assert np.array_equal(got, ref)
     # and not the fixed-omega run's
     assert not np.array_equal(got, oracle.sor_rb_f32(p, R, None, itermax=N, tol=-1.0)[0])
@@ -123,10 +125,6 @@ def test_schedule_refusals(qesmc, oracle):
     p = _city(oracle, qesmc, 34, 28, 24, 9)
     with qesmc.Context(0) as ctx:
         _load(ctx, p)
-        for path in (qesmc.PATH_PLAIN, qesmc.PATH_FUSED):
-            with pytest.raises(qesmc.QesmcError) as e:
-                ctx.run(itermax=5, chebyshev_rho=0.9, path=path)
-            assert e.value.code == qesmc.ERR_UNSUPPORTED
         for bad in (dict(chebyshev_rho=1.0), dict(chebyshev_rho=-0.1), dict(chebyshev_rho=float("nan")),
                     dict(chebyshev_rho=0.9, chebyshev_start=-1)):
             with pytest.raises(qesmc.QesmcError) as e:

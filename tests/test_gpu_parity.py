@@ -250,6 +250,13 @@ def test_city_1000x1000x150_bit_exact(oracle, qesmc):
             assert_same(u, ou, "u")
             assert_same(v, ov, "v")
             assert_same(w, ow, "w")
+    # the Chebyshev schedule on the march: a factor per colour within its launch
+    R = oracle.divergence_f32(p)
+    sched = oracle.sor_rb_sched_f32(p, R, oracle.chebyshev_omegas(0.99999, 2 * 12))
+    with qesmc.Context(0) as ctx:
+        u, v, w, lam, res = ctx.solve(p, itermax=12, tol=-1.0, chebyshev_rho=0.99999)
+        assert res["path"] == FUSED and res["iterations"] == 12
+        assert_same(lam, sched, "lambda, Chebyshev schedule on the march")
     # a no-allocation run chooses the march only once its buffers exist
     with qesmc.Context(0) as ctx:
         ctx.set_geometry(p)
