@@ -273,6 +273,36 @@ def test_city_1000x1000x150_bit_exact(oracle, qesmc):
         assert_same(ctx.get_lambda(), lam, "refined lambda, march against split")
 
 
+def test_ragged_city_beyond_the_infinity_cache_auto_is_the_split_result(qesmc):
+    """703 x 517 x 97 cells (35 M; colour arrays 331 MB, beyond the 256 MB
+    Infinity Cache): QESMC_PATH_AUTO runs the two-colour march (x-tiles of
+    uneven width, a partial last k-chunk, 64-row segments) and returns the
+    split kernel's lambda and field bit for bit (the split kernel is checked
+    against the oracle above), plain and with refine_every, to a tolerance
+    that stops mid-run."""
+    dz = qesmc.dz_geometric(97, 1.0, 1.02)
+    blds = qesmc.random_buildings(909, 703, 517, 1.0, 900, 6.0, 16.0, 20.0, 0.6, lognormal=True, hcap=80.0,
+                                  margin=2)
+    with qesmc.Context(0) as ctx:
+        ctx.set_city(703, 517, 1.0, dz, blds, (1, 4.0, 10.0, 0.5, 233.0))
+        # a tolerance the reference's rule reaches by iteration 60
+        ctx.set_lambda(None)
+        tol = float(np.float32(ctx.run(itermax=60, tol=-1.0)["final_max_dlambda"])) * 1.0001
+        for kw in (dict(itermax=23, tol=-1.0), dict(itermax=41, tol=-1.0, omega=1.95, refine_every=10),
+                   dict(itermax=400, tol=tol)):
+            out = {}
+            for path in (AUTO, SPLIT):
+                ctx.set_lambda(None)
+                r = ctx.run(path=path, **kw)
+                out[path] = (r, ctx.get_lambda(), ctx.get_velocity())
+            assert out[AUTO][0]["path"] == FUSED and out[SPLIT][0]["path"] == SPLIT
+            assert out[AUTO][0]["iterations"] == out[SPLIT][0]["iterations"]
+            assert_same(out[AUTO][1], out[SPLIT][1], f"lambda {kw}")
+            for a, b, n in zip(out[AUTO][2], out[SPLIT][2], "uvw"):
+                assert_same(a, b, f"{n} {kw}")
+        assert out[AUTO][0]["iterations"] <= 60
+
+
 @pytest.mark.parametrize("tile", [(1, 1, 1), (3, 1, 2), (8, 2, 8), (16, 4, 4), (5, 4, 16), (64, 2, 1)])
 def test_tile_parameters_never_change_results(gpu_ctx, oracle, qesmc, tile):
     """(tile_k, tile_v, wg_per_cu) reshape the work decomposition only."""
