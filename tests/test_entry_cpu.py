@@ -84,6 +84,22 @@ def test_roofline_arithmetic():
     assert roof["kernel"] == "k_sor_split<true, 2>" and roof["traffic"] is None
 
 
+def test_roofline_arithmetic_of_the_march():
+    """The two-colour march: one launch per iteration over every updated cell
+    at 13 B per update, and its committed traffic figure on the benchmark
+    city."""
+    b = _bench()
+    Q = b.Q
+    res = [{"iterations": 500, "path": Q.PATH_FUSED, "compressed": 1, "tile_k": 8, "tile_v": 1,
+            "ms_sor": 283.5}] * 3
+    roof, ms = b.roofline_of(res, 148_404_596, "city1000x1000x150")
+    launch_s = 283.5 / 500 / 1000.0
+    assert roof["kernel"] == "k_sor_march<8>" and roof["bytes_per_lup"] == 13.0
+    assert roof["updates_per_launch"] == 148_404_596 and roof["avg_launch_us"] == 567.0
+    assert abs(roof["achieved"] - 13.0 * 148_404_596 / launch_s / 1e9) < 0.1
+    assert roof["traffic"] == b.load_traffic("k_sor_march<8>", "city1000x1000x150")[0] > 0
+
+
 def test_traffic_entries_name_their_summaries():
     """Every counter figure the bench line can carry comes from a committed
     profiles/ summary."""
